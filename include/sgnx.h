@@ -1,0 +1,161 @@
+/*
+ * sgnx.h -- C-ABI of libsgnx: MI355X (gfx950) HIP implementation of the 2D
+ * flux-reconstruction (P = 4, Gauss-Lobatto) right-hand side of the hyperbolic
+ * Serre-Green-Naghdi system and of its adaptive Bogacki-Shampine 3(2) step.
+ *
+ * Plain C types only (pointers, sizes, doubles): every entry point returns an
+ * int status (SGNX_OK = 0) and never aborts; sgnx_last_error() gives the text.
+ * A solver handle is single-threaded and not re-entrant, like the reference's
+ * device class. Each entry point below names the reference interface it
+ * replaces (paths relative to the reference checkout).
+ *
+ * State layout (unchanged from the reference, src/solver/sgn2d.h:22-27):
+ *   U[v*Nd + e*25 + a + 5*b],  v in {h, hu, hv, hw, hp},  e = ex + Nx*ey,
+ *   node (a, b) of element e,  Nd = 25*Nx*Ny nodes per variable.
+ */
+#ifndef SGNX_H
+#define SGNX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SGNX_OK 0
+#define SGNX_EINVAL 1       /* bad argument or descriptor */
+#define SGNX_EDEVICE 2      /* HIP runtime error */
+#define SGNX_ENOMEM 3       /* device allocation failed */
+#define SGNX_EUNSUPPORTED 4 /* valid for the reference but not implemented here */
+
+/* side codes, side order (left, right, bottom, top) */
+#define SGNX_BC_PERIODIC 0  /* the reference's only kind (src/core/mesh2d.h:36-45) */
+#define SGNX_BC_WALL 1      /* reflective: ghost = face state, normal momentum negated */
+#define SGNX_BC_INFLOW 2    /* prescribed ghost (left side only) */
+
+/* where the pointers passed to sgnx_rhs live */
+#define SGNX_MEM_HOST 0
+#define SGNX_MEM_DEVICE 1
+
+/* Physical constants; the same four fields and meaning as sgn2d::Params
+ * (src/physics/sgn_flux2d.h:57-62). */
+typedef struct sgnx_params {
+  double g, c, gamma, nManning;
+} sgnx_params;
+
+/* Everything the device solver needs, taken from the host setup; the reference
+ * device class reads the same items from its host SGN2D object
+ * (src/cuda/sgn2d_cuda.cu:318-400). All pointers are host pointers and are
+ * copied during sgnx_create. */
+typedef struct sgnx_desc {
+  int P;                       /* polynomial degree; the kernels implement P = 4 */
+  int Nx, Ny;                  /* elements in x and y (>= 2) */
+  double Lx, Ly;               /* domain size; hx = Lx/Nx, Jx = hx/2 (mesh2d.h:29-50) */
+  sgnx_params par;
+  const double *D;             /* (P+1)^2 row-major differentiation matrix */
+  const double *lL, *lR;       /* face interpolation rows (must be GLL unit rows) */
+  const double *gLp, *gRp;     /* FR correction derivatives at the nodes */
+  const double *qw;            /* quadrature weights (mass, energy) */
+  const double *Hnod;          /* Nd nodal still depth, or NULL = flat H == 0 */
+  const double *Hx, *Hy;       /* Nd gradients, or NULL = computed on the device */
+  /* still depth on every interface line (NULL with Hnod == NULL):
+   *   HfaceX[(ey*(Nx+1) + i)*(P+1) + t]  x-interface i = 0..Nx, line node t
+   *   HfaceY[(j*Nx + ex)*(P+1) + t]      y-interface j = 0..Ny, line node t
+   * For a periodic axis the last line repeats the first (the reference samples
+   * its right/top edge depth at the wrapped coordinate, sgn2d.cpp:75-83). */
+  const double *HfaceX, *HfaceY;
+  int bc[4];                   /* SGNX_BC_* for left, right, bottom, top */
+  /* SGNX_BC_INFLOW on the left side: per face node (ey, t) and variable v the
+   * ghost is (base + s*sin(w t)) + c*cos(w t); table [(ey*(P+1)+t)*15 + 3v + {0,1,2}]. */
+  const double *inflow;
+  double inflow_omega;
+} sgnx_desc;
+
+/* AdaptiveRKOptions (src/solver/rk.h:53-61) plus max_accepted (0 = off): stop
+ * after that many accepted steps, so a fixed step count runs in ONE call and
+ * keeps the PI controller memory. */
+typedef struct sgnx_rk_opts {
+  double atol, rtol, dtMin, dtMax, safety, facMin, facMax;
+  long max_accepted;
+} sgnx_rk_opts;
+
+/* AdaptiveRKStats (src/solver/rk.h:64-72). */
+typedef struct sgnx_rk_stats {
+  long accepted, rejected, rhsEvals;
+  double tReached, dtLast, maxNormLast;
+  int aborted;
+} sgnx_rk_stats;
+
+typedef struct sgnx_solver sgnx_solver;
+
+/* Text of the last failure on this handle; with s == NULL, of the last
+ * sgnx_create on the calling thread. */
+const char* sgnx_last_error(const sgnx_solver* s);
+
+/* Replaces SGN2DCuda::SGN2DCuda(const SGN2D&) (src/cuda/sgn2d_cuda.h:33,
+ * sgn2d_cuda.cu:318-400): allocates the device state, stages and tables. */
+int sgnx_create(const sgnx_desc* d, sgnx_solver** out);
+/* Replaces SGN2DCuda::~SGN2DCuda (sgn2d_cuda.cu:403-412). */
+int sgnx_destroy(sgnx_solver* s);
+/* Stream for all work of this handle (a hipStream_t; NULL = default stream).
+ * Not in the reference, which uses the legacy default stream only. */
+int sgnx_set_stream(sgnx_solver* s, void* stream);
+/* Nodes per variable Nd (SGN2DCuda::nDof, sgn2d_cuda.h:38). */
+long long sgnx_ndof(const sgnx_solver* s);
+/* Device address of the resident state (5*Nd doubles), for zero-copy callers. */
+void* sgnx_state_ptr(sgnx_solver* s);
+
+/* Replace SGN2DCuda::upload / download (sgn2d_cuda.h:41-42). n = 5*Nd. */
+int sgnx_upload(sgnx_solver* s, const double* U, size_t n);
+int sgnx_download(const sgnx_solver* s, double* U, size_t n);
+
+/* Replaces SGN2D::rhs(U, dUdt) (src/solver/sgn2d.cpp:125-290) and
+ * SGN2DCuda::rhsDevice (sgn2d_cuda.cu:429-452): dU = R(U, t), bitwise equal to
+ * the CPU rhs() for finite U. t only enters inflow ghosts. mem = SGNX_MEM_HOST
+ * (copied through the device) or SGNX_MEM_DEVICE (both pointers on this
+ * device, U and dU distinct). */
+int sgnx_rhs(sgnx_solver* s, const double* U, double* dU, size_t n, double t, int mem);
+/* Replaces SGN2DCuda::rhsOnce (sgn2d_cuda.cu:456-462): RHS of the resident
+ * state into host dU. */
+int sgnx_rhs_once(sgnx_solver* s, double* dU, size_t n, double t);
+
+/* Replaces SGN2DCuda::integrateAdaptive (sgn2d_cuda.cu:467-540) with the
+ * arithmetic of sgn::integrateAdaptive (src/solver/rk.cpp:67-164) on the
+ * resident state. step_log (optional, capacity cap steps) receives (dt,
+ * errNorm) of every attempted step. */
+int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgnx_rk_opts* o,
+                   sgnx_rk_stats* st, double* step_log, long cap);
+/* Replace sgn::rk4Step / sgn::bs3Step (src/solver/rk.cpp:23-54) on the
+ * resident state. */
+int sgnx_rk4_step(sgnx_solver* s, double t, double dt);
+int sgnx_bs3_step(sgnx_solver* s, double t, double dt);
+
+/* Replace SGN2D::maxWaveSpeed / cflTimeStep / mass / energy
+ * (sgn2d.cpp:295-350) on the resident state. maxWaveSpeed and cflTimeStep are
+ * bitwise; mass and energy are fixed-order tree sums. */
+int sgnx_max_wave_speed(sgnx_solver* s, double* out);
+int sgnx_cfl_time_step(sgnx_solver* s, double cfl, double* out);
+int sgnx_mass(sgnx_solver* s, double* out);
+int sgnx_energy(sgnx_solver* s, double* out);
+/* The still-depth gradients the solver holds (Nd each, host pointers). */
+int sgnx_bathymetry(const sgnx_solver* s, double* Hx, double* Hy);
+
+/* ---- host setup (no device needed) ---------------------------------------- */
+
+/* Replaces sgn::FROperators1D(P, GaussLobatto) (src/core/fr_operators.cpp:134-200):
+ * nodes x, D, lL, lR, gL', gR', quadrature weights, each P+1 (D (P+1)^2). */
+int sgnx_fr_operators(int P, double* x, double* D, double* lL, double* lR, double* gLp,
+                      double* gRp, double* qw);
+/* Replaces SGN2D::nodeCoords (src/solver/sgn2d.cpp:89-102): Nd each. */
+int sgnx_node_coords(int P, int Nx, int Ny, double Lx, double Ly, double* x, double* y);
+/* Sample points of HfaceX / HfaceY (layouts above), as sgn2d.cpp:75-83; with
+ * wrapx / wrapy the last interface line uses the wrapped coordinate 0. */
+int sgnx_face_coords(int P, int Nx, int Ny, double Lx, double Ly, int wrapx, int wrapy,
+                     double* fxx, double* fxy, double* fyx, double* fyy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGNX_H */

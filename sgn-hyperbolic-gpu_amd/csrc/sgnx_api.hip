@@ -1,0 +1,433 @@
+// =============================================================================
+// sgnx_api.hip -- the C-ABI of libsgnx: solver handles, device memory, and the
+// device-resident adaptive Bogacki-Shampine 3(2) loop.
+//
+// The loop is the arithmetic of src/solver/rk.cpp:67-164 on device buffers:
+// stage inputs, the third-order candidate and the scaled embedded error are
+// formed by kernels with the CPU's association (not the reference GPU
+// kComb3's), the error partials are folded on the device in a fixed order, and
+// only the (sum, max) pair crosses to the host, where the PI controller runs
+// with the C library's pow as in rk.cpp:149-161. The RHS and stage arithmetic
+// are bitwise equal to the CPU's; the error sum differs from the CPU's
+// sequential sum only in rounding, so accept/reject decisions agree unless an
+// error norm lies within rounding of 1.
+// =============================================================================
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sgnx.h"
+#include "sgnx_internal.h"
+
+using namespace sgnx;
+
+struct sgnx_solver {
+  int P = 4, Nx = 0, Ny = 0;
+  double Lx = 0.0, Ly = 0.0, hx = 0.0, hy = 0.0;
+  Geometry g{};
+  Phys ph{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // device buffers
+  OpTables* d_ops = nullptr;
+  double *d_Hnod = nullptr, *d_Hx = nullptr, *d_Hy = nullptr;
+  double *d_HfX = nullptr, *d_HfY = nullptr, *d_inflow = nullptr;
+  double *d_U = nullptr, *d_k1 = nullptr, *d_k2 = nullptr, *d_k3 = nullptr, *d_k4 = nullptr;
+  double *d_tmp = nullptr, *d_uNew = nullptr;
+  double *d_part = nullptr, *d_out = nullptr;
+  double* h_out = nullptr;   // pinned, 2 doubles
+  double inflowOmega = 0.0;
+  bool hasInflow = false;
+  std::string err;
+
+  size_t n() const { return (size_t)NV * (size_t)g.Nd; }
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(sgnx_solver* s, int code, const std::string& msg) {
+  if (s) s->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(s, call)                                                               \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail((s), e_ == hipErrorOutOfMemory ? SGNX_ENOMEM : SGNX_EDEVICE,        \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                 \
+  } while (0)
+
+void free_all(sgnx_solver* s) {
+  for (void* p : {(void*)s->d_ops, (void*)s->d_Hnod, (void*)s->d_Hx, (void*)s->d_Hy,
+                  (void*)s->d_HfX, (void*)s->d_HfY, (void*)s->d_inflow, (void*)s->d_U,
+                  (void*)s->d_k1, (void*)s->d_k2, (void*)s->d_k3, (void*)s->d_k4,
+                  (void*)s->d_tmp, (void*)s->d_uNew, (void*)s->d_part, (void*)s->d_out})
+    if (p) (void)hipFree(p);
+  if (s->h_out) (void)hipHostFree(s->h_out);
+}
+
+GhostTime ghost_time(const sgnx_solver* s, double t) {
+  GhostTime gt{0.0, 0.0};
+  if (s->hasInflow) {
+    gt.s = std::sin(s->inflowOmega * t);
+    gt.c = std::cos(s->inflowOmega * t);
+  }
+  return gt;
+}
+
+int rhs_dev(sgnx_solver* s, const double* U, double* dU, double t) {
+  HIPCHK(s, launch_rhs(s->g, s->ph, U, dU, ghost_time(s, t), s->stream));
+  return SGNX_OK;
+}
+
+int sync(sgnx_solver* s) {
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return SGNX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sgnx_last_error(const sgnx_solver* s) {
+  return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
+  if (!d || !out) return fail(nullptr, SGNX_EINVAL, "null descriptor or output");
+  *out = nullptr;
+  if (d->P != 4) return fail(nullptr, SGNX_EUNSUPPORTED, "the kernels implement P = 4 only");
+  if (d->Nx < 2 || d->Ny < 2) return fail(nullptr, SGNX_EINVAL, "SGN2D: Nx,Ny >= 2");
+  if (!(d->Lx > 0.0) || !(d->Ly > 0.0)) return fail(nullptr, SGNX_EINVAL, "Lx, Ly must be > 0");
+  if (!d->D || !d->lL || !d->lR || !d->gLp || !d->gRp || !d->qw)
+    return fail(nullptr, SGNX_EINVAL, "operator tables missing");
+  for (int k = 0; k < NP; ++k) {
+    const double eL = (k == 0) ? 1.0 : 0.0, eR = (k == NP - 1) ? 1.0 : 0.0;
+    if (d->lL[k] != eL || d->lR[k] != eR || std::signbit(d->lL[k]) || std::signbit(d->lR[k]))
+      return fail(nullptr, SGNX_EUNSUPPORTED,
+                  "face rows are not the Gauss-Lobatto unit rows (only GLL points are supported)");
+  }
+  for (int k = 0; k < 4; ++k)
+    if (d->bc[k] < 0 || d->bc[k] > 2) return fail(nullptr, SGNX_EINVAL, "bad side code");
+  if ((d->bc[0] == SGNX_BC_PERIODIC) != (d->bc[1] == SGNX_BC_PERIODIC) ||
+      (d->bc[2] == SGNX_BC_PERIODIC) != (d->bc[3] == SGNX_BC_PERIODIC))
+    return fail(nullptr, SGNX_EINVAL, "periodic sides must come in pairs");
+  if (d->bc[1] == SGNX_BC_INFLOW || d->bc[2] == SGNX_BC_INFLOW || d->bc[3] == SGNX_BC_INFLOW)
+    return fail(nullptr, SGNX_EUNSUPPORTED, "inflow is supported on the left side only");
+  if (d->bc[0] == SGNX_BC_INFLOW && !d->inflow)
+    return fail(nullptr, SGNX_EINVAL, "inflow side without an inflow table");
+  if (d->Hnod && (!d->HfaceX || !d->HfaceY))
+    return fail(nullptr, SGNX_EINVAL, "face still depths required with a nodal still depth");
+
+  auto* s = new (std::nothrow) sgnx_solver();
+  if (!s) return fail(nullptr, SGNX_ENOMEM, "host allocation failed");
+  s->P = d->P; s->Nx = d->Nx; s->Ny = d->Ny; s->Lx = d->Lx; s->Ly = d->Ly;
+  s->hx = d->Lx / d->Nx;
+  s->hy = d->Ly / d->Ny;
+  Geometry& g = s->g;
+  g.Nx = d->Nx; g.Ny = d->Ny;
+  g.Nd = (long long)d->Nx * d->Ny * NPP;
+  g.Jx = 0.5 * s->hx;                    // mesh2d.h:49-50
+  g.Jy = 0.5 * s->hy;
+  for (int k = 0; k < 4; ++k) g.bc[k] = d->bc[k];
+  Phys& ph = s->ph;
+  ph.g = d->par.g; ph.c = d->par.c; ph.gamma = d->par.gamma; ph.nManning = d->par.nManning;
+  ph.halfg = 0.5 * ph.g;
+  ph.c2 = ph.c * ph.c;
+  ph.mc2 = -ph.c * ph.c;
+  ph.m2c2 = -2.0 * ph.c2;
+  s->hasInflow = d->bc[0] == SGNX_BC_INFLOW;
+  s->inflowOmega = d->inflow_omega;
+
+  auto cleanup_fail = [&](int code) {
+    std::string msg = s->err;
+    free_all(s);
+    delete s;
+    return fail(nullptr, code, msg);
+  };
+#define CK(call)                                                                   \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      s->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+      return cleanup_fail(e_ == hipErrorOutOfMemory ? SGNX_ENOMEM : SGNX_EDEVICE); \
+    }                                                                              \
+  } while (0)
+
+  CK(hipGetDevice(&s->device));
+  const size_t Nd = (size_t)g.Nd, nst = (size_t)NV * Nd;
+  const size_t nfx = (size_t)(d->Nx + 1) * d->Ny * NP, nfy = (size_t)d->Nx * (d->Ny + 1) * NP;
+  OpTables T;
+  std::memcpy(T.D, d->D, sizeof(T.D));
+  std::memcpy(T.gLp, d->gLp, sizeof(T.gLp));
+  std::memcpy(T.gRp, d->gRp, sizeof(T.gRp));
+  std::memcpy(T.qw, d->qw, sizeof(T.qw));
+  CK(hipMalloc((void**)&s->d_ops, sizeof(OpTables)));
+  CK(hipMemcpy(s->d_ops, &T, sizeof(OpTables), hipMemcpyHostToDevice));
+  CK(hipMalloc((void**)&s->d_Hnod, Nd * sizeof(double)));
+  CK(hipMalloc((void**)&s->d_Hx, Nd * sizeof(double)));
+  CK(hipMalloc((void**)&s->d_Hy, Nd * sizeof(double)));
+  CK(hipMalloc((void**)&s->d_HfX, nfx * sizeof(double)));
+  CK(hipMalloc((void**)&s->d_HfY, nfy * sizeof(double)));
+  g.ops = s->d_ops;
+  g.Hnod = s->d_Hnod; g.Hx = s->d_Hx; g.Hy = s->d_Hy; g.HfX = s->d_HfX; g.HfY = s->d_HfY;
+  if (d->Hnod) {
+    CK(hipMemcpy(s->d_Hnod, d->Hnod, Nd * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->d_HfX, d->HfaceX, nfx * sizeof(double), hipMemcpyHostToDevice));
+    CK(hipMemcpy(s->d_HfY, d->HfaceY, nfy * sizeof(double), hipMemcpyHostToDevice));
+    if (d->Hx && d->Hy) {
+      CK(hipMemcpy(s->d_Hx, d->Hx, Nd * sizeof(double), hipMemcpyHostToDevice));
+      CK(hipMemcpy(s->d_Hy, d->Hy, Nd * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      CK(launch_bathy_grad(g, s->d_Hnod, s->d_Hx, s->d_Hy, nullptr));
+      CK(hipDeviceSynchronize());
+    }
+  } else {
+    CK(hipMemset(s->d_Hnod, 0, Nd * sizeof(double)));
+    CK(hipMemset(s->d_Hx, 0, Nd * sizeof(double)));
+    CK(hipMemset(s->d_Hy, 0, Nd * sizeof(double)));
+    CK(hipMemset(s->d_HfX, 0, nfx * sizeof(double)));
+    CK(hipMemset(s->d_HfY, 0, nfy * sizeof(double)));
+  }
+  if (s->hasInflow) {
+    const size_t ni = (size_t)d->Ny * NP * NV * 3;
+    CK(hipMalloc((void**)&s->d_inflow, ni * sizeof(double)));
+    CK(hipMemcpy(s->d_inflow, d->inflow, ni * sizeof(double), hipMemcpyHostToDevice));
+  }
+  g.inflow = s->d_inflow;
+  for (double** p : {&s->d_U, &s->d_k1, &s->d_k2, &s->d_k3, &s->d_k4, &s->d_tmp, &s->d_uNew})
+    CK(hipMalloc((void**)p, nst * sizeof(double)));
+  CK(hipMemset(s->d_U, 0, nst * sizeof(double)));
+  CK(hipMalloc((void**)&s->d_part, 3 * kReduceBlocks * sizeof(double)));
+  CK(hipMalloc((void**)&s->d_out, 2 * sizeof(double)));
+  CK(hipHostMalloc((void**)&s->h_out, 2 * sizeof(double), hipHostMallocDefault));
+#undef CK
+  *out = s;
+  return SGNX_OK;
+}
+
+int sgnx_destroy(sgnx_solver* s) {
+  if (!s) return SGNX_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  else (void)hipDeviceSynchronize();
+  free_all(s);
+  delete s;
+  return SGNX_OK;
+}
+
+int sgnx_set_stream(sgnx_solver* s, void* stream) {
+  if (!s) return SGNX_EINVAL;
+  s->stream = static_cast<hipStream_t>(stream);
+  return SGNX_OK;
+}
+
+long long sgnx_ndof(const sgnx_solver* s) { return s ? s->g.Nd : 0; }
+
+void* sgnx_state_ptr(sgnx_solver* s) { return s ? s->d_U : nullptr; }
+
+int sgnx_upload(sgnx_solver* s, const double* U, size_t n) {
+  if (!s || !U) return SGNX_EINVAL;
+  if (n != s->n()) return fail(s, SGNX_EINVAL, "upload: size mismatch");
+  HIPCHK(s, hipMemcpyAsync(s->d_U, U, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  return sync(s);
+}
+
+int sgnx_download(const sgnx_solver* cs, double* U, size_t n) {
+  auto* s = const_cast<sgnx_solver*>(cs);
+  if (!s || !U) return SGNX_EINVAL;
+  if (n != s->n()) return fail(s, SGNX_EINVAL, "download: size mismatch");
+  HIPCHK(s, hipMemcpyAsync(U, s->d_U, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  return sync(s);
+}
+
+int sgnx_rhs(sgnx_solver* s, const double* U, double* dU, size_t n, double t, int mem) {
+  if (!s || !U || !dU) return SGNX_EINVAL;
+  if (n != s->n()) return fail(s, SGNX_EINVAL, "SGN2D::rhs: size mismatch");
+  if (mem == SGNX_MEM_DEVICE) {
+    if (U == dU) return fail(s, SGNX_EINVAL, "rhs: U and dU must be distinct");
+    return rhs_dev(s, U, dU, t);
+  }
+  if (mem != SGNX_MEM_HOST) return fail(s, SGNX_EINVAL, "rhs: bad memory kind");
+  HIPCHK(s, hipMemcpyAsync(s->d_tmp, U, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  int rc = rhs_dev(s, s->d_tmp, s->d_k2, t);
+  if (rc) return rc;
+  HIPCHK(s, hipMemcpyAsync(dU, s->d_k2, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  return sync(s);
+}
+
+int sgnx_rhs_once(sgnx_solver* s, double* dU, size_t n, double t) {
+  if (!s || !dU) return SGNX_EINVAL;
+  if (n != s->n()) return fail(s, SGNX_EINVAL, "rhsOnce: size mismatch");
+  int rc = rhs_dev(s, s->d_U, s->d_k1, t);
+  if (rc) return rc;
+  HIPCHK(s, hipMemcpyAsync(dU, s->d_k1, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  return sync(s);
+}
+
+int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgnx_rk_opts* o,
+                   sgnx_rk_stats* st, double* step_log, long cap) {
+  if (!s || !o || !st) return SGNX_EINVAL;
+  // rk.cpp:67-164
+  const int orderLow = 2;
+  const double expo = 1.0 / (orderLow + 1.0);
+  const double alpha = 0.7 * expo;
+  const double beta = 0.4 * expo;
+  sgnx_rk_stats S{};
+  S.tReached = t0;
+  const size_t n = s->n();
+  double t = t0;
+  double dt = std::min(dt0, o->dtMax);
+  double errPrev = 1.0;
+  int rc = rhs_dev(s, s->d_U, s->d_k1, t);   // FSAL head
+  if (rc) return rc;
+  ++S.rhsEvals;
+  long attempt = 0;
+  while (t < tEnd - 1e-15 * std::max(1.0, std::abs(tEnd))) {
+    if (o->max_accepted > 0 && S.accepted >= o->max_accepted) break;
+    if (t + dt > tEnd) dt = tEnd - t;
+    HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
+    if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
+    HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.75 * dt, s->d_k2, n, s->stream));
+    if ((rc = rhs_dev(s, s->d_tmp, s->d_k3, t + 0.75 * dt))) return rc;
+    HIPCHK(s, launch_bs3_update(s->d_uNew, s->d_U, dt, s->d_k1, s->d_k2, s->d_k3, n, s->stream));
+    if ((rc = rhs_dev(s, s->d_uNew, s->d_k4, t + dt))) return rc;
+    S.rhsEvals += 3;
+    HIPCHK(s, launch_err_norm(s->d_U, s->d_uNew, s->d_k1, s->d_k2, s->d_k3, s->d_k4, dt, o->atol,
+                              o->rtol, n, s->d_part, s->d_out, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->h_out, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                             s->stream));
+    if ((rc = sync(s))) return rc;
+    const double errSum = s->h_out[0], errMax = s->h_out[1];
+    const double errL2 = std::sqrt(errSum / static_cast<double>(n));
+    const double errNorm = std::max(errL2, 1e-300);
+    if (step_log && attempt < cap) {
+      step_log[2 * attempt] = dt;
+      step_log[2 * attempt + 1] = errNorm;
+    }
+    ++attempt;
+    if (!(errNorm == errNorm)) {
+      // a non-finite error norm would make the controller's dt NaN and the loop
+      // endless; stop with the state at the last accepted step instead
+      S.aborted = 1;
+      S.dtLast = dt;
+      break;
+    }
+    if (errNorm <= 1.0) {
+      t += dt;
+      std::swap(s->d_U, s->d_uNew);
+      std::swap(s->d_k1, s->d_k4);
+      ++S.accepted;
+      S.tReached = t;
+      S.dtLast = dt;
+      S.maxNormLast = errMax;
+      errPrev = errNorm;
+    } else {
+      ++S.rejected;
+    }
+    double fac = o->safety * std::pow(errNorm, -alpha) * std::pow(errPrev, beta);
+    fac = std::clamp(fac, o->facMin, o->facMax);
+    double dtNext = dt * fac;
+    dtNext = std::min(dtNext, o->dtMax);
+    if (dtNext < o->dtMin) {
+      S.aborted = 1;
+      S.dtLast = dtNext;
+      break;
+    }
+    dt = dtNext;
+  }
+  if ((rc = sync(s))) return rc;
+  *st = S;
+  return SGNX_OK;
+}
+
+int sgnx_rk4_step(sgnx_solver* s, double t, double dt) {
+  if (!s) return SGNX_EINVAL;
+  // rk.cpp:23-38
+  const size_t n = s->n();
+  int rc;
+  if ((rc = rhs_dev(s, s->d_U, s->d_k1, t))) return rc;
+  HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
+  if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
+  HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k2, n, s->stream));
+  if ((rc = rhs_dev(s, s->d_tmp, s->d_k3, t + 0.5 * dt))) return rc;
+  HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, dt, s->d_k3, n, s->stream));
+  if ((rc = rhs_dev(s, s->d_tmp, s->d_k4, t + dt))) return rc;
+  HIPCHK(s, launch_rk4_update(s->d_U, dt, s->d_k1, s->d_k2, s->d_k3, s->d_k4, n, s->stream));
+  return sync(s);
+}
+
+int sgnx_bs3_step(sgnx_solver* s, double t, double dt) {
+  if (!s) return SGNX_EINVAL;
+  // rk.cpp:41-54
+  const size_t n = s->n();
+  int rc;
+  if ((rc = rhs_dev(s, s->d_U, s->d_k1, t))) return rc;
+  HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
+  if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
+  HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.75 * dt, s->d_k2, n, s->stream));
+  if ((rc = rhs_dev(s, s->d_tmp, s->d_k3, t + 0.75 * dt))) return rc;
+  HIPCHK(s, launch_bs3_update(s->d_uNew, s->d_U, dt, s->d_k1, s->d_k2, s->d_k3, n, s->stream));
+  std::swap(s->d_U, s->d_uNew);
+  return sync(s);
+}
+
+static int diag(sgnx_solver* s, int which, double* out) {
+  if (!s || !out) return SGNX_EINVAL;
+  HIPCHK(s, launch_diag(s->g, s->ph, s->d_U, which, s->d_part, s->d_out, s->stream));
+  HIPCHK(s, hipMemcpyAsync(s->h_out, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           s->stream));
+  int rc = sync(s);
+  if (rc) return rc;
+  *out = (which == 0) ? s->h_out[1] : s->h_out[0];
+  return SGNX_OK;
+}
+
+int sgnx_max_wave_speed(sgnx_solver* s, double* out) { return diag(s, 0, out); }
+
+int sgnx_cfl_time_step(sgnx_solver* s, double cfl, double* out) {
+  double lam = 0.0;
+  int rc = diag(s, 0, &lam);
+  if (rc) return rc;
+  // sgn2d.cpp:310-317
+  const double dxEff = std::fmin(s->hx, s->hy) / (2.0 * s->P + 1.0);
+  *out = cfl * dxEff / (lam > 0.0 ? lam : 1e-300);
+  return SGNX_OK;
+}
+
+int sgnx_mass(sgnx_solver* s, double* out) {
+  double m = 0.0;
+  int rc = diag(s, 1, &m);
+  if (rc) return rc;
+  *out = m * (s->g.Jx * s->g.Jy);
+  return SGNX_OK;
+}
+
+int sgnx_energy(sgnx_solver* s, double* out) {
+  double E = 0.0;
+  int rc = diag(s, 2, &E);
+  if (rc) return rc;
+  *out = E * (s->g.Jx * s->g.Jy);
+  return SGNX_OK;
+}
+
+int sgnx_bathymetry(const sgnx_solver* cs, double* Hx, double* Hy) {
+  auto* s = const_cast<sgnx_solver*>(cs);
+  if (!s || !Hx || !Hy) return SGNX_EINVAL;
+  const size_t Nd = (size_t)s->g.Nd;
+  HIPCHK(s, hipMemcpyAsync(Hx, s->d_Hx, Nd * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipMemcpyAsync(Hy, s->d_Hy, Nd * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  return sync(s);
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// =============================================================================
+// sgnx_internal.h -- device-side data of a solver handle and the kernel
+// launchers shared by the C-ABI layer (sgnx_api.hip) and the kernels
+// (sgnx_kernels.hip).
+// =============================================================================
+#ifndef SGNX_INTERNAL_H
+#define SGNX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "sgnx_phys.h"
+
+namespace sgnx {
+
+constexpr int NP = 5;     // nodes per direction at P = 4
+constexpr int NPP = 25;   // nodes per element
+
+// Per-lane operator tables (uploaded once): D row-major, gL', gR'.
+struct OpTables {
+  double D[NPP];
+  double gLp[NP], gRp[NP];
+  double qw[NP];
+};
+
+// What one RHS launch needs besides the stage input and output.
+struct Geometry {
+  int Nx, Ny;
+  long long Nd;              // nodes per variable
+  double Jx, Jy;
+  int bc[4];                 // left, right, bottom, top
+  const OpTables* ops;       // device
+  const double* Hnod;        // device, Nd
+  const double* Hx;          // device, Nd
+  const double* Hy;          // device, Nd
+  const double* HfX;         // device, (Nx+1)*Ny*5
+  const double* HfY;         // device, Nx*(Ny+1)*5
+  const double* inflow;      // device, Ny*5*15 or null
+};
+
+// Inflow ghost time factors for one RHS evaluation: sin(w t), cos(w t),
+// evaluated on the host with the C library, as the oracle does.
+struct GhostTime { double s, c; };
+
+// ---- launchers (sgnx_kernels.hip); all asynchronous on `st` ----------------
+
+// dU = R(U)
+hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double* dU,
+                      GhostTime gt, hipStream_t st);
+// Hx, Hy = (D . Hnod) / J per element line (sgn2d.cpp:54-71)
+hipError_t launch_bathy_grad(const Geometry& g, const double* Hnod, double* Hx, double* Hy,
+                             hipStream_t st);
+// out = u + a1*k1
+hipError_t launch_comb1(double* out, const double* u, double a1, const double* k1, size_t n,
+                        hipStream_t st);
+// out = u + dt*((2/9 k1 + 1/3 k2) + 4/9 k3)   (rk.cpp:106-108)
+hipError_t launch_bs3_update(double* out, const double* u, double dt, const double* k1,
+                             const double* k2, const double* k3, size_t n, hipStream_t st);
+// u += (dt/6)*(((k1 + 2k2) + 2k3) + k4)        (rk.cpp:36-37)
+hipError_t launch_rk4_update(double* u, double dt, const double* k1, const double* k2,
+                             const double* k3, const double* k4, size_t n, hipStream_t st);
+// Scaled embedded error (rk.cpp:118-131): out[0] = sum r^2, out[1] = max |r|,
+// reduced in a fixed order with compensated sums; `part` holds
+// 3*kReduceBlocks doubles of scratch.
+constexpr int kReduceBlocks = 1024;
+hipError_t launch_err_norm(const double* u, const double* uNew, const double* k1,
+                           const double* k2, const double* k3, const double* k4, double dt,
+                           double atol, double rtol, size_t n, double* part, double* out,
+                           hipStream_t st);
+// Diagnostics of one state: which = 0 max wave speed (bitwise), 1 mass, 2 energy.
+hipError_t launch_diag(const Geometry& g, const Phys& ph, const double* U, int which,
+                       double* part, double* out, hipStream_t st);
+
+}  // namespace sgnx
+
+#endif  // SGNX_INTERNAL_H

@@ -1,0 +1,496 @@
+// =============================================================================
+// sgnx_kernels.hip -- gfx950 kernels of the FR/SGN right-hand side and of the
+// Bogacki-Shampine 3(2) stage, update and error-norm passes.
+//
+// RHS (one launch, replaces the five phases of src/solver/sgn2d.cpp:125-290):
+//   A workgroup owns a tile of TW consecutive elements of one element row
+//   (25*TW nodes, one work-item per node; the state layout keeps each
+//   variable's element blocks contiguous, so the tile is one coalesced span
+//   per variable). Three phases separated by workgroup barriers:
+//     0  load U, H at the node; primitives; nodal fluxes F, G -> LDS
+//     1  one work-item per interface node of the tile (TW+1 x-lines, TW
+//        bottom and TW top y-lines): Rusanov flux, central depth h*, and the
+//        two one-sided fluxes, which are the elements' own face fluxes
+//     2  one work-item per node: D.F and D.G from LDS, FR lift of the face
+//        jumps, FR depth gradients, non-conservative products, source, dU
+//   Face extrapolation on GLL points is the edge node itself: the reference's
+//   sum 0 + 1*v0 + 0*v1 + ... equals v0 + 0.0 for finite values (the +0.0
+//   turns -0 into +0 as that sum does), so it is never materialised.
+//   Neighbour face nodes outside the tile are read from global memory; the
+//   domain sides are periodic (wrapped index), walls (mirror ghost) or the
+//   left inflow ghost.
+//   Tiles are handed out XCD-contiguously: consecutive workgroup ids land on
+//   different XCDs, so ids are remapped such that each XCD sweeps its own
+//   contiguous band of tiles and finds the row below still in its L2.
+//
+// Every floating-point expression keeps the reference's evaluation order; the
+// file is compiled with -ffp-contract=off.
+// =============================================================================
+
+#include "sgnx_internal.h"
+
+namespace sgnx {
+
+namespace {
+
+constexpr int kThreads = 256;     // elementwise / reduction kernels
+
+template <int TW>
+struct Tile {
+  static constexpr int NT = ((NPP * TW + 63) / 64) * 64;  // work-items (whole waves)
+  static constexpr int NXI = NP * (TW + 1);               // x-interface nodes
+  static constexpr int NYI = NP * TW;                     // y-interface nodes per side
+};
+
+// Bijective remap of workgroup ids so that XCD x (ids x, x+8, x+16, ...)
+// receives the contiguous tile range [start_x, start_x + count_x).
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb - q * kXcd;
+  const int x = bid - (bid / kXcd) * kXcd, i = bid / kXcd;
+  return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Node (a, b) of element (ex, ey), variable v (sgn2d.h:22-27, mesh2d.h:31).
+__device__ __forceinline__ long long nidx(const Geometry& g, int ex, int ey, int a, int b) {
+  return ((long long)ey * g.Nx + ex) * NPP + a + NP * b;
+}
+
+// Face state of a node: the GLL extrapolation value v + 0.0 (see header).
+__device__ __forceinline__ void load_face(const double* __restrict__ U, long long Nd,
+                                          long long node, double* W) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) W[v] = U[(long long)v * Nd + node] + 0.0;
+}
+
+// Ghost state across a non-periodic side.
+__device__ __forceinline__ void wall_ghost(const double* In, int normal, double* G) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) G[v] = In[v];
+  G[normal] = -In[normal];
+}
+
+template <int TW>
+__global__ void __launch_bounds__(Tile<TW>::NT)
+k_rhs(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph, GhostTime gt,
+      int tilesPerRow) {
+  constexpr int NT = Tile<TW>::NT, NXI = Tile<TW>::NXI, NYI = Tile<TW>::NYI;
+  __shared__ double sU[NV][NT];
+  __shared__ double sF[NV][NT];
+  __shared__ double sG[NV][NT];
+  __shared__ double xFs[NV][NXI], xFl[NV][NXI], xFr[NV][NXI];
+  __shared__ double xhs[NXI], xhl[NXI], xhr[NXI];
+  __shared__ double yFsB[NV][NYI], yFrB[NV][NYI], yhsB[NYI], yhrB[NYI];
+  __shared__ double yFsT[NV][NYI], yFlT[NV][NYI], yhsT[NYI], yhlT[NYI];
+
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ey = tile / tilesPerRow;
+  const int ex0 = (tile - ey * tilesPerRow) * TW;
+  const int nW = min(TW, g.Nx - ex0);
+  const int nNodes = NPP * nW;
+  const int t = threadIdx.x;
+  const long long Nd = g.Nd;
+  const long long gbase = ((long long)ey * g.Nx + ex0) * NPP;
+
+  // ---- phase 0: nodal state and fluxes (sgn2d.cpp:188-196) -----------------
+  double Un[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  Prim q{0.0, 0.0, 0.0, 0.0};
+  double Hn = 0.0;
+  if (t < nNodes) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Un[v] = U[(long long)v * Nd + gbase + t];
+    Hn = g.Hnod[gbase + t];
+    q = prims(Un);
+    double F[NV], G[NV];
+    flux_x(Un, q, Hn, ph, F);
+    flux_y(Un, q, Hn, ph, G);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      sU[v][t] = Un[v];
+      sF[v][t] = F[v];
+      sG[v][t] = G[v];
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 1: interface fluxes (sgn2d.cpp:198-239) -----------------------
+  const int nXI = NP * (nW + 1);
+  if (t < nXI) {
+    const int i = t / NP, b = t - NP * i;
+    const int ex = ex0 + i;                   // this interface is the left edge of column ex
+    double UL[NV], UR[NV];
+    bool ghostL = false, ghostR = false;
+    if (i < nW) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) UR[v] = sU[v][i * NPP + NP * b] + 0.0;
+    } else if (ex < g.Nx) {
+      load_face(U, Nd, nidx(g, ex, ey, 0, b), UR);
+    } else if (g.bc[1] == 0) {
+      load_face(U, Nd, nidx(g, 0, ey, 0, b), UR);
+    } else {
+      ghostR = true;
+    }
+    if (i > 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) UL[v] = sU[v][(i - 1) * NPP + (NP - 1) + NP * b] + 0.0;
+    } else if (ex > 0) {
+      load_face(U, Nd, nidx(g, ex - 1, ey, NP - 1, b), UL);
+    } else if (g.bc[0] == 0) {
+      load_face(U, Nd, nidx(g, g.Nx - 1, ey, NP - 1, b), UL);
+    } else {
+      ghostL = true;
+    }
+    if (ghostL) {
+      if (g.bc[0] == 1) {
+        wall_ghost(UR, IHU, UL);
+      } else {
+        const double* c = g.inflow + ((long long)ey * NP + b) * (NV * 3);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) UL[v] = (c[3 * v] + c[3 * v + 1] * gt.s) + c[3 * v + 2] * gt.c;
+      }
+    }
+    if (ghostR) wall_ghost(UL, IHU, UR);
+    const double Hf = g.HfX[((long long)ey * (g.Nx + 1) + ex) * NP + b];
+    double Fs[NV], FL[NV], FR[NV];
+    interface_flux<0>(UL, UR, Hf, ph, Fs, FL, FR);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      xFs[v][t] = Fs[v];
+      xFl[v][t] = FL[v];
+      xFr[v][t] = FR[v];
+    }
+    xhs[t] = 0.5 * (UL[IH] + UR[IH]);
+    xhl[t] = UL[IH];
+    xhr[t] = UR[IH];
+  } else if (t < nXI + 2 * NP * nW) {
+    const int r2 = t - nXI;
+    const int side = r2 / (NP * nW);          // 0 bottom edge, 1 top edge
+    const int r = r2 - side * NP * nW;
+    const int j = r / NP, a = r - NP * j;
+    const int ex = ex0 + j;
+    double UL[NV], UR[NV];
+    if (side == 0) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) UR[v] = sU[v][j * NPP + a] + 0.0;
+      if (ey > 0) load_face(U, Nd, nidx(g, ex, ey - 1, a, NP - 1), UL);
+      else if (g.bc[2] == 0) load_face(U, Nd, nidx(g, ex, g.Ny - 1, a, NP - 1), UL);
+      else wall_ghost(UR, IHV, UL);
+      const double Hf = g.HfY[((long long)ey * g.Nx + ex) * NP + a];
+      double Fs[NV], FL[NV], FR[NV];
+      interface_flux<1>(UL, UR, Hf, ph, Fs, FL, FR);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        yFsB[v][r] = Fs[v];
+        yFrB[v][r] = FR[v];
+      }
+      yhsB[r] = 0.5 * (UL[IH] + UR[IH]);
+      yhrB[r] = UR[IH];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) UL[v] = sU[v][j * NPP + a + NP * (NP - 1)] + 0.0;
+      if (ey + 1 < g.Ny) load_face(U, Nd, nidx(g, ex, ey + 1, a, 0), UR);
+      else if (g.bc[3] == 0) load_face(U, Nd, nidx(g, ex, 0, a, 0), UR);
+      else wall_ghost(UL, IHV, UR);
+      const double Hf = g.HfY[((long long)(ey + 1) * g.Nx + ex) * NP + a];
+      double Fs[NV], FL[NV], FR[NV];
+      interface_flux<1>(UL, UR, Hf, ph, Fs, FL, FR);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        yFsT[v][r] = Fs[v];
+        yFlT[v][r] = FL[v];
+      }
+      yhsT[r] = 0.5 * (UL[IH] + UR[IH]);
+      yhlT[r] = UL[IH];
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: FR lift and assembly (sgn2d.cpp:241-288) -------------------
+  if (t < nNodes) {
+    const int j = t / NPP, loc = t - NPP * j;
+    const int b = loc / NP, a = loc - NP * b;
+    const OpTables* T = g.ops;
+    double Da[NP], Db[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) { Da[k] = T->D[a * NP + k]; Db[k] = T->D[b * NP + k]; }
+    const double gLa = T->gLp[a], gRa = T->gRp[a], gLb = T->gLp[b], gRb = T->gRp[b];
+    const int ixL = j * NP + b, ixR = (j + 1) * NP + b, iy = j * NP + a;
+    const int rowx = j * NPP + NP * b, coly = j * NPP + a;
+    double out[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double dfx = 0.0, dgy = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        dfx = dfx + Da[k] * sF[v][rowx + k];
+        dgy = dgy + Db[k] * sG[v][coly + NP * k];
+      }
+      const double cx = (xFs[v][ixL] - xFr[v][ixL]) * gLa + (xFs[v][ixR] - xFl[v][ixR]) * gRa;
+      const double cy = (yFsB[v][iy] - yFrB[v][iy]) * gLb + (yFsT[v][iy] - yFlT[v][iy]) * gRb;
+      const double dF = (dfx + cx) / g.Jx;
+      const double dG = (dgy + cy) / g.Jy;
+      out[v] = -dF - dG;
+    }
+    double dhx = 0.0, dhy = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      dhx = dhx + Da[k] * sU[IH][rowx + k];
+      dhy = dhy + Db[k] * sU[IH][coly + NP * k];
+    }
+    dhx = dhx + ((xhs[ixL] - xhr[ixL]) * gLa + (xhs[ixR] - xhl[ixR]) * gRa);
+    dhy = dhy + ((yhsB[iy] - yhrB[iy]) * gLb + (yhsT[iy] - yhlT[iy]) * gRb);
+    const double hx = dhx / g.Jx, hy = dhy / g.Jy;
+    // ncpX / ncpY (sgn_flux2d.h:150-161): only the hp entry is non-zero, and
+    // subtracting the zero entries leaves the other sums unchanged
+    const double Bx = ph.mc2 * q.u * hx;
+    const double By = ph.mc2 * q.v * hy;
+    double S[NV];
+    source(Un, q, Hn, g.Hx[gbase + t], g.Hy[gbase + t], ph, S);
+    out[IHP] = out[IHP] - Bx - By;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) dU[(long long)v * Nd + gbase + t] = out[v] + S[v];
+  }
+}
+
+// Hx, Hy at every node (sgn2d.cpp:54-71).
+__global__ void __launch_bounds__(kThreads)
+k_bathy_grad(const double* __restrict__ H, double* __restrict__ Hx, double* __restrict__ Hy,
+             const OpTables* __restrict__ T, long long Nd, double Jx, double Jy) {
+  const long long gn = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (gn >= Nd) return;
+  const long long e = gn / NPP;
+  const int loc = (int)(gn - e * NPP);
+  const int b = loc / NP, a = loc - NP * b;
+  const double* He = H + e * NPP;
+  double dHx = 0.0, dHy = 0.0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    dHx = dHx + T->D[a * NP + k] * He[k + NP * b];
+    dHy = dHy + T->D[b * NP + k] * He[a + NP * k];
+  }
+  Hx[gn] = dHx / Jx;
+  Hy[gn] = dHy / Jy;
+}
+
+// ---- RK stage arithmetic (src/solver/rk.cpp) ----------------------------------
+
+__global__ void __launch_bounds__(kThreads)
+k_comb1(double* __restrict__ out, const double* __restrict__ u, double a1,
+        const double* __restrict__ k1, size_t n) {
+  const size_t stride = (size_t)gridDim.x * kThreads;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    out[i] = u[i] + a1 * k1[i];
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bs3_update(double* __restrict__ out, const double* __restrict__ u, double dt,
+             const double* __restrict__ k1, const double* __restrict__ k2,
+             const double* __restrict__ k3, size_t n) {
+  const size_t stride = (size_t)gridDim.x * kThreads;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    out[i] = u[i] + dt * (2.0 / 9.0 * k1[i] + 1.0 / 3.0 * k2[i] + 4.0 / 9.0 * k3[i]);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_rk4_update(double* __restrict__ u, double dt, const double* __restrict__ k1,
+             const double* __restrict__ k2, const double* __restrict__ k3,
+             const double* __restrict__ k4, size_t n) {
+  const size_t stride = (size_t)gridDim.x * kThreads;
+  const double d6 = dt / 6.0;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
+    u[i] += d6 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+}
+
+// Compensated accumulation (Knuth's TwoSum): s + c carries the running sum
+// with the rounding error of every addition kept in c.
+__device__ __forceinline__ void acc_add(double& s, double& c, double x) {
+  const double t = s + x;
+  const double bp = t - s;
+  const double e = (s - (t - bp)) + (x - bp);
+  s = t;
+  c = c + e;
+}
+
+// Fixed-order block reduction of (s, c, m) triples held in LDS.
+__device__ __forceinline__ void block_reduce(double* ss, double* sc, double* sm) {
+  const int t = threadIdx.x;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      double s = ss[t], c = sc[t];
+      acc_add(s, c, ss[t + off]);
+      ss[t] = s;
+      sc[t] = c + sc[t + off];
+      sm[t] = fmax(sm[t], sm[t + off]);
+    }
+    __syncthreads();
+  }
+}
+
+// rk.cpp:118-131: r = (uNew - uEmb) / (atol + rtol*max(|u|, |uNew|)).
+__global__ void __launch_bounds__(kThreads)
+k_err_norm(const double* __restrict__ u, const double* __restrict__ uNew,
+           const double* __restrict__ k1, const double* __restrict__ k2,
+           const double* __restrict__ k3, const double* __restrict__ k4, double dt, double atol,
+           double rtol, size_t n, double* __restrict__ part) {
+  __shared__ double ss[kThreads], sc[kThreads], sm[kThreads];
+  double s = 0.0, c = 0.0, m = 0.0;
+  const size_t stride = (size_t)gridDim.x * kThreads;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const double uEmb =
+        u[i] + dt * (7.0 / 24.0 * k1[i] + 0.25 * k2[i] + 1.0 / 3.0 * k3[i] + 0.125 * k4[i]);
+    const double e = uNew[i] - uEmb;
+    const double au = fabs(u[i]), an = fabs(uNew[i]);
+    const double sc_ = atol + rtol * ((au < an) ? an : au);   // std::max(|u|, |uNew|)
+    const double r = e / sc_;
+    acc_add(s, c, r * r);
+    const double ar = fabs(r);
+    m = (m < ar) ? ar : m;                                     // std::max(errMax, |r|)
+  }
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  sm[threadIdx.x] = m;
+  block_reduce(ss, sc, sm);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = ss[0];
+    part[3 * blockIdx.x + 1] = sc[0];
+    part[3 * blockIdx.x + 2] = sm[0];
+  }
+}
+
+// Second level: one block folds the nb partial triples in a fixed order.
+// out[0] = sum, out[1] = max.
+__global__ void __launch_bounds__(kThreads)
+k_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double ss[kThreads], sc[kThreads], sm[kThreads];
+  double s = 0.0, c = 0.0, m = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) {
+    acc_add(s, c, part[3 * i]);
+    c = c + part[3 * i + 1];
+    m = fmax(m, part[3 * i + 2]);
+  }
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  sm[threadIdx.x] = m;
+  block_reduce(ss, sc, sm);
+  if (threadIdx.x == 0) {
+    out[0] = ss[0] + sc[0];
+    out[1] = sm[0];
+  }
+}
+
+// Diagnostics over nodes: 0 = max wave speed (sgn2d.cpp:295-304), 1 = mass
+// integrand qw_a qw_b h (:319-332), 2 = energy integrand (:334-350).
+__global__ void __launch_bounds__(kThreads)
+k_diag(const double* __restrict__ U, Geometry g, Phys ph, int which, double* __restrict__ part) {
+  __shared__ double ss[kThreads], sc[kThreads], sm[kThreads];
+  double s = 0.0, c = 0.0, m = 0.0;
+  const long long Nd = g.Nd;
+  const size_t stride = (size_t)gridDim.x * kThreads;
+  for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < (size_t)Nd; i += stride) {
+    double Un[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) Un[v] = U[(long long)v * Nd + i];
+    if (which == 0) {
+      const Prim q = prims(Un);
+      const double ce = signal_speed(Un[IH], q, ph);
+      const double s0 = fabs(q.u) + ce, s1 = fabs(q.v) + ce;
+      m = fmax(m, fmax(s0, s1));
+    } else {
+      const int loc = (int)(i % NPP);
+      const int b = loc / NP, a = loc - NP * b;
+      const double wq = g.ops->qw[a] * g.ops->qw[b];
+      double val;
+      if (which == 1) {
+        val = wq * Un[IH];
+      } else {
+        const double h = Un[IH];
+        const double u = Un[IHU] / h, v = Un[IHV] / h, w = Un[IHW] / h;
+        const double ed = 0.5 * h *
+                          (u * u + v * v + (1.0 / ph.gamma) * w * w + ph.g * (h - 2.0 * g.Hnod[i]));
+        val = wq * ed;
+      }
+      acc_add(s, c, val);
+    }
+  }
+  ss[threadIdx.x] = s;
+  sc[threadIdx.x] = c;
+  sm[threadIdx.x] = m;
+  block_reduce(ss, sc, sm);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = ss[0];
+    part[3 * blockIdx.x + 1] = sc[0];
+    part[3 * blockIdx.x + 2] = sm[0];
+  }
+}
+
+inline unsigned elementwise_blocks(size_t n) {
+  const size_t want = (n + kThreads - 1) / kThreads;
+  return (unsigned)(want < 16384 ? (want ? want : 1) : 16384);
+}
+
+constexpr int kTileW = 10;   // elements per RHS tile
+
+}  // namespace
+
+hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double* dU,
+                      GhostTime gt, hipStream_t st) {
+  const int tpr = (g.Nx + kTileW - 1) / kTileW;
+  const long long tiles = (long long)tpr * g.Ny;
+  if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_rhs<kTileW>, dim3((unsigned)tiles), dim3(Tile<kTileW>::NT), 0, st, U, dU,
+                     g, ph, gt, tpr);
+  return hipGetLastError();
+}
+
+hipError_t launch_bathy_grad(const Geometry& g, const double* Hnod, double* Hx, double* Hy,
+                             hipStream_t st) {
+  const long long nb = (g.Nd + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_bathy_grad, dim3((unsigned)nb), dim3(kThreads), 0, st, Hnod, Hx, Hy,
+                     g.ops, g.Nd, g.Jx, g.Jy);
+  return hipGetLastError();
+}
+
+hipError_t launch_comb1(double* out, const double* u, double a1, const double* k1, size_t n,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_comb1, dim3(elementwise_blocks(n)), dim3(kThreads), 0, st, out, u, a1, k1,
+                     n);
+  return hipGetLastError();
+}
+
+hipError_t launch_bs3_update(double* out, const double* u, double dt, const double* k1,
+                             const double* k2, const double* k3, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_bs3_update, dim3(elementwise_blocks(n)), dim3(kThreads), 0, st, out, u,
+                     dt, k1, k2, k3, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_rk4_update(double* u, double dt, const double* k1, const double* k2,
+                             const double* k3, const double* k4, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_rk4_update, dim3(elementwise_blocks(n)), dim3(kThreads), 0, st, u, dt, k1,
+                     k2, k3, k4, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_err_norm(const double* u, const double* uNew, const double* k1,
+                           const double* k2, const double* k3, const double* k4, double dt,
+                           double atol, double rtol, size_t n, double* part, double* out,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_err_norm, dim3(kReduceBlocks), dim3(kThreads), 0, st, u, uNew, k1, k2, k3,
+                     k4, dt, atol, rtol, n, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, st, part, kReduceBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag(const Geometry& g, const Phys& ph, const double* U, int which,
+                       double* part, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag, dim3(kReduceBlocks), dim3(kThreads), 0, st, U, g, ph, which, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, st, part, kReduceBlocks, out);
+  return hipGetLastError();
+}
+
+}  // namespace sgnx

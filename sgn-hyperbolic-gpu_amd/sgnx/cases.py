@@ -1,0 +1,254 @@
+"""Problem builders: still depths, initial states and boundary data of the
+workloads the solver is measured and checked on.
+
+Each builder returns a `Case` with the constructor arguments of SGN2D and the
+initial state; the same arrays feed the GPU solver and (in the tests) the
+oracle, so any libm difference between machines is shared by both.
+
+Workloads
+  gate       8x6 elements on 10x8 m over a Gaussian bump with a smooth
+             (h,u,v,w,p) state (the reference's GPU-vs-CPU gate,
+             test/test_cuda_sgn2d.cu:52-64)
+  solitary   128x64 on 64x32 m, flat 1 m still depth, sech^2 solitary wave
+             a/h0 = 0.1 with its crest at x = -16 m (x = 16 m in mesh
+             coordinates), periodic in x and walls in y (formulas of
+             apps/shoal_focus_mpi.cu:117-137)
+  shoal      1024x512 on 96x48 m: 1:20 slope 0.45 -> 0.10 m plus a seeded
+             elliptic Gaussian shoal, 1 s / 0.01 m incident wave on the left
+             (slow-branch linear eigenmode), walls elsewhere, fluid at rest
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import Params
+from .solver import node_coords
+
+G = 9.81
+GAMMA = 1.5
+ALPHA = 3.0
+
+
+@dataclass
+class Case:
+    name: str
+    P: int
+    Nx: int
+    Ny: int
+    Lx: float
+    Ly: float
+    par: Params
+    depth: object            # dict of arrays or None
+    bc: tuple
+    U0: np.ndarray
+    inflow: object = None    # (table, omega) or None
+    meta: dict = field(default_factory=dict)
+
+    def solver_kwargs(self):
+        return dict(P=self.P, Nx=self.Nx, Ny=self.Ny, Lx=self.Lx, Ly=self.Ly, par=self.par,
+                    still_depth=self.depth, bc=self.bc, inflow=self.inflow)
+
+    @property
+    def nd(self):
+        return self.Nx * self.Ny * (self.P + 1) ** 2
+
+
+def _face_points(P, Nx, Ny, Lx, Ly, wrapx, wrapy):
+    from .solver import face_coords
+    return face_coords(P, Nx, Ny, Lx, Ly, wrapx, wrapy)
+
+
+def depth_arrays(P, Nx, Ny, Lx, Ly, fn, bc):
+    """Still depth at nodes and on every interface line for H = fn(x, y)."""
+    x, y = node_coords(P, Nx, Ny, Lx, Ly)
+    fxx, fxy, fyx, fyy = _face_points(P, Nx, Ny, Lx, Ly, bc[0] == "periodic", bc[2] == "periodic")
+    return {"Hnod": np.ascontiguousarray(fn(x, y), dtype=np.float64),
+            "HfaceX": np.ascontiguousarray(fn(fxx, fxy), dtype=np.float64),
+            "HfaceY": np.ascontiguousarray(fn(fyx, fyy), dtype=np.float64)}
+
+
+def prim_to_state(h, u, v, w, p):
+    """fromPrimitive (src/physics/sgn_flux2d.h:65-68), var-major."""
+    return np.concatenate([h, h * u, h * v, h * w, h * p]).astype(np.float64)
+
+
+# ---------------------------------------------------------------------------
+def gate(P=4, Nx=8, Ny=6, bc=("periodic",) * 4):
+    Lx, Ly = 10.0, 8.0
+    par = Params(g=G, c=ALPHA * math.sqrt(G * 1.0), gamma=GAMMA)
+
+    def bump(x, y):
+        dx = x - 0.5 * Lx
+        dy = y - 0.5 * Ly
+        return 1.0 - 0.4 * np.exp(-(dx * dx + dy * dy) / 6.0)
+
+    depth = depth_arrays(P, Nx, Ny, Lx, Ly, bump, bc)
+    x, y = node_coords(P, Nx, Ny, Lx, Ly)
+    kx, ky = 2.0 * math.pi / Lx, 2.0 * math.pi / Ly
+    h = depth["Hnod"] + 0.04 * np.sin(kx * x) * np.cos(ky * y)
+    u = 0.08 * np.cos(kx * x)
+    v = -0.05 * np.sin(ky * y)
+    w = 0.02 * np.sin(kx * x)
+    p = 0.01 * np.cos(ky * y)
+    return Case("gate", P, Nx, Ny, Lx, Ly, par, depth, tuple(bc), prim_to_state(h, u, v, w, p))
+
+
+# ---------------------------------------------------------------------------
+def solitary_state(x, H, h0, a, g, gamma, x0):
+    """Travelling SGN solitary wave (h, u, w, p) riding on still depth H."""
+    cs = math.sqrt(g * (h0 + a))
+    kap = math.sqrt(3.0 * a / (4.0 * h0 * h0 * (h0 + a)))
+    th = np.tanh(kap * (x - x0))
+    s = 1.0 - th * th
+    hs = h0 + a * s
+    hx = -2.0 * kap * a * s * th
+    hxx = 4.0 * kap * kap * a * s * th * th - 2.0 * kap * kap * a * s * s
+    h = H + a * s
+    u = cs * (1.0 - h0 / hs)
+    w = -cs * h0 * hx / (2.0 * hs)
+    p = -(cs * h0 / gamma) * (-0.5 * cs * h0 * (hxx * hs - hx * hx) / (hs * hs))
+    return h, u, w, p
+
+
+def solitary(Nx=128, Ny=64, walls=True, P=4):
+    Lx, Ly = 64.0, 32.0
+    h0, a = 1.0, 0.1
+    par = Params(g=G, c=ALPHA * math.sqrt(G * h0), gamma=GAMMA)
+    bc = ("periodic", "periodic", "wall", "wall") if walls else ("periodic",) * 4
+    depth = depth_arrays(P, Nx, Ny, Lx, Ly, lambda x, y: np.full_like(x, h0), bc)
+    x, _ = node_coords(P, Nx, Ny, Lx, Ly)
+    h, u, w, p = solitary_state(x, depth["Hnod"], h0, a, G, GAMMA, x0=16.0)
+    U0 = prim_to_state(h, u, np.zeros_like(h), w, p)
+    return Case("solitary", P, Nx, Ny, Lx, Ly, par, depth, bc, U0,
+                meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "steps": 200})
+
+
+# ---------------------------------------------------------------------------
+def slow_cp2(kH, gamma, alpha):
+    """Slow-branch phase speed^2/(gH) of the relaxed system, Escalante-Dumbser-
+    Castro (2019) eq. (15) with the + root."""
+    J = 1.0 + kH * kH / (2.0 * gamma)
+    mu = (kH / alpha) * (kH / alpha) / (2.0 * gamma)
+    half = 0.5 * (J + mu)
+    disc = half * half - mu
+    return 1.0 / (half + math.sqrt(disc if disc > 0.0 else 0.0))
+
+
+def wavenumber(omega, H, g=G, gamma=GAMMA, alpha=ALPHA):
+    """k with k * Cp(k) = omega on the slow branch. The slow-branch frequency
+    k*Cp(k) increases with k, so bisection on [0, k_hi] converges to the
+    double nearest the root without a derivative."""
+    def f(k):
+        return k * math.sqrt(slow_cp2(k * H, gamma, alpha) * g * H) - omega
+    lo, hi = 0.0, omega / math.sqrt(g * H)
+    while f(hi) < 0.0:
+        hi *= 2.0
+        if hi * H > 1e6:
+            # k*Cp(k) saturates at sqrt(2 gamma g / (H (1 + 1/alpha^2))) for large k
+            raise ValueError(f"no slow-branch wave of frequency {omega} at depth {H}")
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mid <= lo or mid >= hi:
+            break
+        if f(mid) < 0.0:
+            lo = mid
+        else:
+            hi = mid
+    return hi if abs(f(hi)) < abs(f(lo)) else lo
+
+
+def inflow_table(Ny, P, H_left, period, amp, g=G, gamma=GAMMA, alpha=ALPHA):
+    """Left-side ghost coefficients of the linear slow-branch eigenmode
+        eta = A sin(w t),  u = (Cp/H) eta,  p = (Cp^2/H - g) eta,
+        w = -(gamma (Cp^2/H - g) A / (w H)) cos(w t)
+    (x = 0, phase k x - w t shifted by a quarter period), as linearised
+    conserved variables h = H + eta, hu = H u, hv = 0, hw = H w, hp = H p.
+    Table layout [(ey*(P+1) + t)*15 + 3*var + {base, sin, cos}]."""
+    omega = 2.0 * math.pi / period
+    np_ = P + 1
+    tab = np.zeros((Ny * np_, 5, 3))
+    Hl = np.broadcast_to(np.asarray(H_left, dtype=np.float64), (Ny * np_,))
+    for i in range(Ny * np_):
+        H = float(Hl[i])
+        k = wavenumber(omega, H, g, gamma, alpha)
+        cp = omega / k
+        Pamp = cp * cp / H - g
+        tab[i, 0] = (H, amp, 0.0)
+        tab[i, 1] = (0.0, cp * amp, 0.0)
+        tab[i, 3] = (0.0, 0.0, -gamma * Pamp * amp / omega)
+        tab[i, 4] = (0.0, H * Pamp * amp, 0.0)
+    return tab.reshape(-1), omega
+
+
+def shoal_params(seed):
+    rng = np.random.default_rng(seed)
+    return {
+        "xc": float(rng.uniform(18.0, 30.0)),
+        "yc": float(rng.uniform(14.0, 34.0)),
+        "ax": float(rng.uniform(1.5, 6.0)),
+        "ay": float(rng.uniform(1.5, 6.0)),
+        "crest": float(rng.uniform(0.3, 0.9)),
+    }
+
+
+def slope_depth(x, h_deep=0.45, h_shelf=0.10, x_toe=40.0, slope=1.0 / 20.0):
+    """1:20 linear slope from h_deep (x <= x_toe) to h_shelf."""
+    return np.clip(h_deep - slope * (x - x_toe), h_shelf, h_deep)
+
+
+def shoal(seed=0, Nx=1024, Ny=512, P=4, period=1.0, amp=0.01):
+    Lx, Ly = 96.0, 48.0
+    h_deep = 0.45
+    par = Params(g=G, c=ALPHA * math.sqrt(G * h_deep), gamma=GAMMA)
+    sp = shoal_params(seed)
+
+    def depth_fn(x, y):
+        base = slope_depth(x)
+        local = slope_depth(np.full_like(x, sp["xc"]))
+        r2 = ((x - sp["xc"]) / sp["ax"]) ** 2 + ((y - sp["yc"]) / sp["ay"]) ** 2
+        return base - sp["crest"] * local * np.exp(-r2)
+
+    bc = ("inflow", "wall", "wall", "wall")
+    depth = depth_arrays(P, Nx, Ny, Lx, Ly, depth_fn, bc)
+    # the ghost sits on the x = 0 interface line; its depth is the face depth there
+    HfX = depth["HfaceX"].reshape(Ny, Nx + 1, P + 1)[:, 0, :].reshape(-1)
+    table, omega = inflow_table(Ny, P, HfX, period, amp)
+    Hn = depth["Hnod"]
+    z = np.zeros_like(Hn)
+    U0 = prim_to_state(Hn.copy(), z, z, z, z)
+    return Case("shoal", P, Nx, Ny, Lx, Ly, par, depth, bc, U0, inflow=(table, omega),
+                meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "steps": 500, "seed": seed,
+                      "shoal": sp, "period": period, "amp": amp})
+
+
+# ---------------------------------------------------------------------------
+def random_state(nd, seed, h0=1.0, amp=1.0):
+    """Node-wise random state with h in h0*[0.6, 1.4] (exercises every branch
+    of the arithmetic; not a smooth field)."""
+    rng = np.random.default_rng(seed)
+    h = h0 * (1.0 + 0.4 * amp * rng.uniform(-1.0, 1.0, nd))
+    u = 0.5 * amp * rng.uniform(-1.0, 1.0, nd)
+    v = 0.5 * amp * rng.uniform(-1.0, 1.0, nd)
+    w = 0.1 * amp * rng.uniform(-1.0, 1.0, nd)
+    p = 0.2 * amp * rng.uniform(-1.0, 1.0, nd)
+    return prim_to_state(h, u, v, w, p)
+
+
+def random_depth(P, Nx, Ny, Lx, Ly, seed, bc):
+    """Smooth random still depth in [0.5, 1.5] (a few Gaussian bumps)."""
+    rng = np.random.default_rng(seed)
+    cx = rng.uniform(0.0, Lx, 4)
+    cy = rng.uniform(0.0, Ly, 4)
+    amp = rng.uniform(-0.2, 0.2, 4)
+    wid = rng.uniform(0.1, 0.3, 4) * min(Lx, Ly)
+
+    def fn(x, y):
+        out = np.ones_like(x)
+        for i in range(4):
+            out = out + amp[i] * np.exp(-((x - cx[i]) ** 2 + (y - cy[i]) ** 2) / (wid[i] ** 2))
+        return out
+
+    return depth_arrays(P, Nx, Ny, Lx, Ly, fn, bc)

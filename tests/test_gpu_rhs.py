@@ -1,0 +1,137 @@
+"""GPU parity of the fused RHS kernel against the oracle (bitwise).
+
+Every case builds the same still-depth arrays and state for libsgnx and for
+the oracle and requires dU to agree bit for bit (finite inputs).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_bitwise(got, want, what):
+    g = np.ascontiguousarray(got, dtype=np.float64).view(np.uint64)
+    w = np.ascontiguousarray(want, dtype=np.float64).view(np.uint64)
+    assert g.shape == w.shape, what
+    bad = np.flatnonzero(g != w)
+    if bad.size:
+        i = int(bad[0])
+        a, b = got.ravel()[i], want.ravel()[i]
+        raise AssertionError(f"{what}: {bad.size} of {g.size} entries differ; first at {i}: "
+                             f"{a!r} ({float(a).hex()}) vs {b!r} ({float(b).hex()}), "
+                             f"max abs diff {np.nanmax(np.abs(got - want)):.3e}")
+
+
+def run_pair(case, t=0.0, U=None):
+    import sgnx
+    U = case.U0 if U is None else U
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    o = OL.Oracle.from_case(case)
+    try:
+        return s.rhs(U, t), o.rhs(U, t)
+    finally:
+        s.close()
+        o.close()
+
+
+def test_rhs_gate_case_bitwise():
+    from sgnx import cases
+    got, want = run_pair(cases.gate())
+    assert_bitwise(got, want, "gate 8x6 RHS")
+
+
+@pytest.mark.parametrize("Nx,Ny", [(2, 2), (3, 5), (10, 3), (11, 4), (23, 7), (40, 9)])
+@pytest.mark.parametrize("sides", ["periodic", "walls_y", "walls"])
+def test_rhs_random_bitwise(Nx, Ny, sides):
+    from sgnx import cases
+    bc = {"periodic": ("periodic",) * 4, "walls_y": ("periodic", "periodic", "wall", "wall"),
+          "walls": ("wall",) * 4}[sides]
+    Lx, Ly = 1.3 * Nx, 0.7 * Ny          # hx, hy not powers of two
+    depth = cases.random_depth(4, Nx, Ny, Lx, Ly, seed=Nx * 31 + Ny, bc=bc)
+    case = cases.Case("random", 4, Nx, Ny, Lx, Ly, cases.Params(9.81, 3.0 * math.sqrt(9.81), 1.5),
+                      depth, bc, cases.random_state(Nx * Ny * 25, seed=Nx + 100 * Ny))
+    got, want = run_pair(case)
+    assert_bitwise(got, want, f"random {Nx}x{Ny} {sides}")
+
+
+def test_rhs_flat_no_depth_bitwise():
+    """Flat bottom given as no still depth at all (the reference's H == 0)."""
+    from sgnx import cases
+    Nx, Ny = 12, 5
+    case = cases.Case("flat", 4, Nx, Ny, 6.0, 2.5, cases.Params(9.81, 2.0, 2.0), None,
+                      ("periodic",) * 4, cases.random_state(Nx * Ny * 25, seed=3))
+    got, want = run_pair(case)
+    assert_bitwise(got, want, "flat H == 0")
+
+
+def test_rhs_inflow_bitwise():
+    from sgnx import cases
+    Nx, Ny = 21, 6
+    Lx, Ly = 4.0, 1.5
+    bc = ("inflow", "wall", "wall", "wall")
+    depth = {k: 0.45 * v for k, v in cases.random_depth(4, Nx, Ny, Lx, Ly, seed=5, bc=bc).items()}
+    HfX = depth["HfaceX"].reshape(Ny, Nx + 1, 5)[:, 0, :].reshape(-1)
+    table, omega = cases.inflow_table(Ny, 4, HfX, period=1.0, amp=0.01)
+    case = cases.Case("inflow", 4, Nx, Ny, Lx, Ly, cases.Params(9.81, 3.0 * math.sqrt(9.81 * 0.45), 1.5),
+                      depth, bc, cases.random_state(Nx * Ny * 25, seed=9), inflow=(table, omega))
+    for t in (0.0, 0.123, 0.37):
+        got, want = run_pair(case, t=t)
+        assert_bitwise(got, want, f"inflow t={t}")
+
+
+def test_rhs_solitary_128x64_bitwise():
+    from sgnx import cases
+    got, want = run_pair(cases.solitary())
+    assert_bitwise(got, want, "solitary 128x64 walls-in-y RHS")
+
+
+def test_lake_at_rest_gpu():
+    """Well-balancing (test/test_sgn2d.cpp:55-66): h = H, u = v = w = p = 0."""
+    import sgnx
+    from sgnx import cases
+    case = cases.gate()
+    Hn = case.depth["Hnod"]
+    z = np.zeros_like(Hn)
+    U = cases.prim_to_state(Hn.copy(), z, z, z, z)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    R = s.rhs(U)
+    s.close()
+    assert np.abs(R).max() < 1e-12
+
+
+def test_bathymetry_gradients_bitwise():
+    import sgnx
+    from sgnx import cases
+    case = cases.gate()
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    Hx, Hy = s.stillDepthGrad()
+    s.close()
+    o = OL.Oracle.from_case(case)
+    _, oHx, oHy = o.bathymetry()
+    assert_bitwise(Hx, oHx, "Hx")
+    assert_bitwise(Hy, oHy, "Hy")
+
+
+def test_rhs_device_pointers_and_determinism():
+    """sgnx_rhs on device pointers; two evaluations give identical bits."""
+    import torch
+    import sgnx
+    from sgnx import cases
+    case = cases.gate(Nx=17, Ny=9)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    U = torch.from_numpy(case.U0).to("cuda:0")
+    d1 = torch.empty_like(U)
+    d2 = torch.empty_like(U)
+    torch.cuda.synchronize()
+    s.rhs_device(U.data_ptr(), d1.data_ptr())
+    s.rhs_device(U.data_ptr(), d2.data_ptr())
+    torch.cuda.synchronize()
+    a, b = d1.cpu().numpy(), d2.cpu().numpy()
+    s.close()
+    assert_bitwise(a, b, "run-to-run")
+    want = OL.Oracle.from_case(case).rhs(case.U0)
+    assert_bitwise(a, want, "device-pointer RHS")

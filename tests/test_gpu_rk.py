@@ -1,0 +1,104 @@
+"""GPU parity of the Runge-Kutta paths and diagnostics against the oracle.
+
+Fixed-step RK4 / BS3 are bitwise (RHS and stage arithmetic are). The adaptive
+integrator's error sum is a fixed-order device reduction rather than the CPU's
+sequential sum, so trajectories are held to the stated bar: identical accepted
+and rejected step counts and relative L2 distance <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from test_gpu_rhs import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_fixed_steps_bitwise():
+    import sgnx
+    from sgnx import cases
+    case = cases.gate(Nx=13, Ny=6, bc=("periodic", "periodic", "wall", "wall"))
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    o = OL.Oracle.from_case(case)
+    dt = 0.5 * o.cfl_time_step(case.U0, 0.5)
+    s.upload(case.U0)
+    Uo = case.U0.copy()
+    for k in range(3):
+        s.rk4Step(k * dt, dt)
+        Uo = o.rk4_step(Uo, k * dt, dt)
+    assert_bitwise(s.download(), Uo, "3 RK4 steps")
+    s.upload(case.U0)
+    Uo = case.U0.copy()
+    for k in range(3):
+        s.bs3Step(k * dt, dt)
+        Uo = o.bs3_step(Uo, k * dt, dt)
+    assert_bitwise(s.download(), Uo, "3 BS3 steps")
+    s.close()
+
+
+def test_diagnostics():
+    import sgnx
+    from sgnx import cases
+    case = cases.gate()
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    o = OL.Oracle.from_case(case)
+    U = case.U0
+    assert s.maxWaveSpeed(U) == o.max_wave_speed(U)
+    assert s.cflTimeStep(U, 0.5) == o.cfl_time_step(U, 0.5)
+    assert abs(s.mass(U) - o.mass(U)) <= 1e-14 * abs(o.mass(U))
+    assert abs(s.energy(U) - o.energy(U)) <= 1e-13 * abs(o.energy(U))
+    s.close()
+
+
+def _adaptive(case, t_end, opts_kw, max_acc=0, step_cap=4096):
+    import sgnx
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    o = OL.Oracle.from_case(case)
+    dt0 = o.cfl_time_step(case.U0, 0.5)
+    s.upload(case.U0)
+    st, glog = s.integrateAdaptive(0.0, t_end, dt0, sgnx.RkOpts(max_accepted=max_acc, **opts_kw),
+                                   step_log_cap=step_cap)
+    Ug = s.download()
+    s.close()
+    Uo, sto, olog, _, _ = o.integrate(case.U0, 0.0, t_end, dt0,
+                                      OL.OrcOpts(max_accepted=max_acc, **opts_kw),
+                                      step_cap=step_cap)
+    return st, sto, Ug, Uo, glog, olog
+
+
+def test_adaptive_gate_matches_oracle():
+    """The reference's adaptive gate (test/test_cuda_sgn2d.cu:96-128), held to
+    the stricter bar: same step counts, relative L2 <= 1e-12."""
+    from sgnx import cases
+    case = cases.gate()
+    st, sto, Ug, Uo, glog, olog = _adaptive(case, 0.3, dict(atol=1e-10, rtol=1e-8))
+    assert not st.aborted and not sto.aborted
+    assert (st.accepted, st.rejected, st.rhsEvals) == (sto.accepted, sto.rejected, sto.rhsEvals)
+    assert rel_l2(Ug, Uo) <= 1e-12
+    n = st.accepted + st.rejected
+    np.testing.assert_allclose(glog[:n, 0], olog[:n, 0], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(glog[:n, 1], olog[:n, 1], rtol=1e-12, atol=0)
+
+
+def test_adaptive_solitary_200_steps():
+    """Solitary wave, 128x64, walls in y: 200 accepted steps in one call."""
+    from sgnx import cases
+    case = cases.solitary()
+    st, sto, Ug, Uo, glog, olog = _adaptive(case, 1e9, dict(atol=1e-8, rtol=1e-6), max_acc=200)
+    assert (st.accepted, st.rejected, st.rhsEvals) == (sto.accepted, sto.rejected, sto.rhsEvals)
+    assert st.accepted == 200
+    assert rel_l2(Ug, Uo) <= 1e-12
+    assert st.tReached == pytest.approx(sto.tReached, rel=1e-12)
+
+
+def test_adaptive_abort_on_dtmin():
+    """dtMin above the step the controller wants -> aborted, like rk.cpp:155-160."""
+    from sgnx import cases
+    case = cases.gate()
+    st, sto, Ug, Uo, _, _ = _adaptive(case, 0.3, dict(atol=1e-14, rtol=1e-14, dtMin=1e-2))
+    assert st.aborted == 1 and sto.aborted == 1
+    assert (st.accepted, st.rejected) == (sto.accepted, sto.rejected)
