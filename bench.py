@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""Throughput of the FR/SGN right-hand side inside the adaptive BS3(2) loop on
+one MI355X, in the reference's own unit (apps/sgn_bench.cu:138-141,
+README.md:372-376): DOF-RHS-evals/s = 5*Nx*Ny*(P+1)^2 * N_RHS / wall time,
+N_RHS counting the RHS evaluations of rejected steps.
+
+Workload (the single-GPU production share): 1024x512 elements, P = 4, FP64,
+96x48 m, 1:20 slope 0.45 -> 0.10 m plus a seeded elliptic shoal, a 1 s /
+0.01 m incident wave on the left side, walls elsewhere, fluid at rest;
+atol 1e-8, rtol 1e-6, dt0 = cflTimeStep(U0, 0.5).
+
+A "step" is one accepted adaptive step. Method as apps/sgn_bench.cu:111-128:
+a warm-up integration, re-upload of the initial state, then ONE device-resident
+integration of exactly --steps accepted steps, timed between two device
+synchronisations (state resident in HBM before the clock starts).
+
+Also reported:
+  roofline      the RHS kernel alone: algorithmic bytes per launch (104 B per
+                node: U read, dU written, H, Hx, Hy read) over its mean device
+                duration measured with HIP events around every RHS launch of the
+                timed run, against the 8 TB/s HBM3E peak; `traffic` is the
+                measured HBM bytes per launch from the rocprofv3 counter runs
+                summarised in profiles/rhs_traffic.json (null if absent)
+  cpu_baseline  the CPU oracle (the reference path restated op for op) on one
+                host core over a bounded sample of the same workload: the FSAL
+                head plus two accepted steps; the same sample also checks the
+                full-size RHS bitwise and the 2-step trajectory.
+
+usage: python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--seed 0]
+       python bench.py --profile-rhs N    (N RHS launches + a copy of known size,
+                                           for rocprofv3 counter runs)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sgn-hyperbolic-gpu_amd"))
+
+METRIC = "DOF-RHS-evals/s at P=4 1024x512, % HBM roofline; batched and adjoint vs plain"
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES_PER_NODE = 104          # 13 doubles: U (5) + dU (5) + H, Hx, Hy
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_case(args):
+    from sgnx import cases
+    t = time.perf_counter()
+    case = cases.shoal(seed=args.seed, Nx=args.nx, Ny=args.ny)
+    log(f"[bench] case {case.name} {case.Nx}x{case.Ny} P={case.P} built in "
+        f"{time.perf_counter() - t:.1f}s")
+    return case
+
+
+def profile_rhs(args):
+    """RHS launches on device-resident buffers, plus a plain device-to-device
+    copy of known size to calibrate the HBM byte counters."""
+    import numpy as np
+    import sgnx
+    from sgnx import hip
+    case = build_case(args)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    U = hip.DeviceArray.from_host(case.U0)
+    dU = hip.DeviceArray(U.n)
+    hip.synchronize()
+    for _ in range(args.profile_rhs):
+        s.rhs_device(U.address, dU.address)
+    hip.synchronize()
+    a = hip.DeviceArray.from_host(np.ones(args.copy_mib * 131072))
+    b = hip.DeviceArray(a.n)
+    for _ in range(3):
+        b.copy_from(a)
+    hip.synchronize()
+    log(f"[bench] profile: {args.profile_rhs} RHS launches of {case.nd} nodes; "
+        f"3 device copies of {args.copy_mib} MiB")
+    s.close()
+
+
+def cpu_baseline(case, gpu_solver, dt0, args):
+    """The oracle on one core: FSAL head + 2 accepted steps of the same workload."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    import sgnx
+    o = OL.Oracle.from_case(case)
+    t = time.perf_counter()
+    R_cpu = o.rhs(case.U0)
+    t_rhs = time.perf_counter() - t
+    R_gpu = gpu_solver.rhs(case.U0)
+    rhs_bitwise = bool(np.array_equal(R_cpu.view(np.uint64), R_gpu.view(np.uint64)))
+    t = time.perf_counter()
+    Uo, sto, _, _, _ = o.integrate(case.U0, 0.0, 1e9, dt0,
+                                   OL.OrcOpts(atol=1e-8, rtol=1e-6, max_accepted=2))
+    t_int = time.perf_counter() - t
+    gpu_solver.upload(case.U0)
+    stg = gpu_solver.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=2))
+    Ug = gpu_solver.download()
+    rel = float(np.linalg.norm(Ug - Uo) / np.linalg.norm(Uo))
+    dof = 5.0 * case.nd
+    value = dof * sto.rhsEvals / t_int
+    o.close()
+    log(f"[bench] cpu: {sto.rhsEvals} RHS in {t_int:.2f}s ({value / 1e6:.1f} MDOF-RHS/s), "
+        f"single RHS {t_rhs:.2f}s; full-size RHS bitwise={rhs_bitwise}, "
+        f"2-step rel L2={rel:.2e}")
+    return {"value": value, "unit": "DOF-RHS-evals/s", "cores": 1, "kind": "port",
+            "sample": (f"CPU oracle (reference path restated op for op, g++ -O2 "
+                       f"-ffp-contract=off, 1 thread) on the same {case.Nx}x{case.Ny} workload: "
+                       f"FSAL head + 2 accepted BS3(2) steps = {sto.rhsEvals} RHS evaluations "
+                       f"in {t_int:.2f} s"),
+            "check": {"full_size_rhs_bitwise": rhs_bitwise, "steps2_rel_l2": rel,
+                      "steps2_counts_equal": (stg.accepted, stg.rejected, stg.rhsEvals)
+                      == (sto.accepted, sto.rejected, sto.rhsEvals)}}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500, help="accepted adaptive steps timed")
+    ap.add_argument("--warmup", type=int, default=20, help="accepted steps of the warm-up run")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--ny", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--profile-rhs", type=int, default=0)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {max(args.gpus, world)}); "
+              "the multi-GPU decomposition is a design in DESIGN.md, not implemented",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.profile_rhs:
+        profile_rhs(args)
+        return
+
+    import sgnx
+    case = build_case(args)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    s.upload(case.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = dict(atol=case.meta["atol"], rtol=case.meta["rtol"])
+
+    # warm-up (untimed), then reset the state as apps/sgn_bench.cu:120 does
+    t = time.perf_counter()
+    stw = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=args.warmup, **opts))
+    log(f"[bench] warm-up: {stw.accepted} acc / {stw.rejected} rej in "
+        f"{time.perf_counter() - t:.2f}s")
+    s.upload(case.U0)
+    s.enable_timing(True)
+    s.rhs_timing()                                  # clear, and wait for the device
+    t0 = time.perf_counter()
+    st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=args.steps, **opts))
+    wall = time.perf_counter() - t0                  # integrateAdaptive ends with a sync
+    launches, rhs_ms = s.rhs_timing()
+    s.enable_timing(False)
+    if st.aborted or st.accepted != args.steps:
+        log(f"[bench] run did not complete: {st.as_dict()}")
+        sys.exit(1)
+
+    nd = case.nd
+    dof = 5.0 * nd
+    value = dof * st.rhsEvals / wall
+    mean_rhs_s = rhs_ms / launches / 1e3
+    achieved = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "rhs_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("Nx") == case.Nx and tr.get("Ny") == case.Ny:
+            traffic = tr.get("bytes_per_launch")
+    log(f"[bench] timed: {st.accepted} acc / {st.rejected} rej / {st.rhsEvals} RHS in "
+        f"{wall:.3f}s -> {value / 1e9:.2f} GDOF-RHS/s; RHS kernel mean {mean_rhs_s * 1e3:.4f} ms "
+        f"over {launches} launches = {achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(case, s, dt0, args)
+    s.close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-RHS-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / st.accepted,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"shoal {case.Nx}x{case.Ny} P=4 FP64 on 96x48 m: 1:20 slope + seeded "
+                         f"elliptic shoal (seed {args.seed}), 1 s / 0.01 m inflow on the left, "
+                         f"walls elsewhere; {args.steps} accepted adaptive BS3(2) steps, "
+                         f"atol 1e-8 rtol 1e-6"),
+            "Nx": case.Nx, "Ny": case.Ny, "P": case.P, "dof": int(dof),
+            "accepted": st.accepted, "rejected": st.rejected, "rhs_evals": st.rhsEvals,
+            "wall_s": wall, "t_reached": st.tReached,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_rhs (fused FR/SGN right-hand side)",
+            "mean_launch_ms": mean_rhs_s * 1e3,
+            "launches": launches,
+            "alg_bytes_per_launch": ALG_BYTES_PER_NODE * nd,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

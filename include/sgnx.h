@@ -141,6 +141,21 @@ int sgnx_energy(sgnx_solver* s, double* out);
 /* The still-depth gradients the solver holds (Nd each, host pointers). */
 int sgnx_bathymetry(const sgnx_solver* s, double* Hx, double* Hy);
 
+/* RHS kernel choice (tuning, no counterpart in the reference): variant 0 =
+ * one workgroup per 10-element row tile, 1 = row sweep over a strip of 10
+ * element columns and `rows` element rows per workgroup (default 1, rows 16;
+ * the environment variables SGNX_RHS_VARIANT / SGNX_SWEEP_ROWS set the
+ * defaults at creation). Both are bitwise equal to the reference rhs(). */
+int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows);
+
+/* Launch timing (instrumentation, no counterpart in the reference): while
+ * enabled, every RHS launch of this handle is bracketed by HIP events on the
+ * handle's stream. sgnx_rhs_timing waits for the stream, returns the number of
+ * RHS launches recorded and their summed device duration in milliseconds, and
+ * clears the record. */
+int sgnx_enable_timing(sgnx_solver* s, int on);
+int sgnx_rhs_timing(sgnx_solver* s, long* launches, double* total_ms);
+
 /* ---- host setup (no device needed) ---------------------------------------- */
 
 /* Replaces sgn::FROperators1D(P, GaussLobatto) (src/core/fr_operators.cpp:134-200):

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -46,6 +47,10 @@ struct sgnx_solver {
   double inflowOmega = 0.0;
   bool hasInflow = false;
   std::string err;
+  // launch timing: event pairs around RHS launches while enabled
+  bool timing = false;
+  std::vector<hipEvent_t> events;   // 2 per recorded launch, reused
+  long timed = 0;
 
   size_t n() const { return (size_t)NV * (size_t)g.Nd; }
 };
@@ -75,6 +80,8 @@ void free_all(sgnx_solver* s) {
                   (void*)s->d_tmp, (void*)s->d_uNew, (void*)s->d_part, (void*)s->d_out})
     if (p) (void)hipFree(p);
   if (s->h_out) (void)hipHostFree(s->h_out);
+  for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+  s->events.clear();
 }
 
 GhostTime ghost_time(const sgnx_solver* s, double t) {
@@ -87,7 +94,20 @@ GhostTime ghost_time(const sgnx_solver* s, double t) {
 }
 
 int rhs_dev(sgnx_solver* s, const double* U, double* dU, double t) {
+  if (!s->timing) {
+    HIPCHK(s, launch_rhs(s->g, s->ph, U, dU, ghost_time(s, t), s->stream));
+    return SGNX_OK;
+  }
+  const size_t need = 2 * (size_t)(s->timed + 1);
+  while (s->events.size() < need) {
+    hipEvent_t e;
+    HIPCHK(s, hipEventCreate(&e));
+    s->events.push_back(e);
+  }
+  HIPCHK(s, hipEventRecord(s->events[2 * s->timed], s->stream));
   HIPCHK(s, launch_rhs(s->g, s->ph, U, dU, ghost_time(s, t), s->stream));
+  HIPCHK(s, hipEventRecord(s->events[2 * s->timed + 1], s->stream));
+  ++s->timed;
   return SGNX_OK;
 }
 
@@ -140,6 +160,12 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   g.Nd = (long long)d->Nx * d->Ny * NPP;
   g.Jx = 0.5 * s->hx;                    // mesh2d.h:49-50
   g.Jy = 0.5 * s->hy;
+  g.dx = make_const_div(g.Jx);
+  g.dy = make_const_div(g.Jy);
+  g.variant = 1;
+  g.rows = 16;
+  if (const char* v = std::getenv("SGNX_RHS_VARIANT")) g.variant = std::atoi(v) == 0 ? 0 : 1;
+  if (const char* r = std::getenv("SGNX_SWEEP_ROWS")) g.rows = std::max(1, std::atoi(r));
   for (int k = 0; k < 4; ++k) g.bc[k] = d->bc[k];
   Phys& ph = s->ph;
   ph.g = d->par.g; ph.c = d->par.c; ph.gamma = d->par.gamma; ph.nManning = d->par.nManning;
@@ -418,6 +444,36 @@ int sgnx_energy(sgnx_solver* s, double* out) {
   int rc = diag(s, 2, &E);
   if (rc) return rc;
   *out = E * (s->g.Jx * s->g.Jy);
+  return SGNX_OK;
+}
+
+int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows) {
+  if (!s || variant < 0 || variant > 1 || rows < 1) return SGNX_EINVAL;
+  s->g.variant = variant;
+  s->g.rows = rows;
+  return SGNX_OK;
+}
+
+int sgnx_enable_timing(sgnx_solver* s, int on) {
+  if (!s) return SGNX_EINVAL;
+  s->timing = on != 0;
+  s->timed = 0;
+  return SGNX_OK;
+}
+
+int sgnx_rhs_timing(sgnx_solver* s, long* launches, double* total_ms) {
+  if (!s || !launches || !total_ms) return SGNX_EINVAL;
+  int rc = sync(s);
+  if (rc) return rc;
+  double sum = 0.0;
+  for (long i = 0; i < s->timed; ++i) {
+    float ms = 0.0f;
+    HIPCHK(s, hipEventElapsedTime(&ms, s->events[2 * i], s->events[2 * i + 1]));
+    sum += ms;
+  }
+  *launches = s->timed;
+  *total_ms = sum;
+  s->timed = 0;
   return SGNX_OK;
 }
 

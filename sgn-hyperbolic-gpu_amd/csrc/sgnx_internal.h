@@ -10,6 +10,7 @@
 
 #include <cstddef>
 
+#include "sgnx_fastdiv.h"
 #include "sgnx_phys.h"
 
 namespace sgnx {
@@ -37,6 +38,9 @@ struct Geometry {
   const double* HfX;         // device, (Nx+1)*Ny*5
   const double* HfY;         // device, Nx*(Ny+1)*5
   const double* inflow;      // device, Ny*5*15 or null
+  ConstDiv dx, dy;           // division by Jx, Jy
+  int variant;               // RHS kernel: 0 element-row tiles, 1 row sweep
+  int rows;                  // row sweep: element rows per workgroup
 };
 
 // Inflow ghost time factors for one RHS evaluation: sin(w t), cos(w t),

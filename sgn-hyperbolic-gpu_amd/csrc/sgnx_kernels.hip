@@ -252,6 +252,264 @@ k_rhs(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph
   }
 }
 
+// ---- RHS, row sweep ---------------------------------------------------------
+//
+// A workgroup owns a strip of TW element columns and a chunk of `rows`
+// element rows and walks up the chunk one element row r per iteration:
+//   A  the row's nodes (prefetched into registers during the previous
+//      iteration): primitives, nodal F and G -> LDS, next row's prefetch
+//      issued; the y-interface items issue the loads of the next row's bottom
+//      face nodes
+//   B  interface items: the TW+1 x-interface lines of row r, and the
+//      y-interface line r+1 on top of the row (the line at its bottom was
+//      done in the previous iteration, or before the loop). A y-interface is
+//      the x-interface arithmetic applied to the states with hu and hv
+//      exchanged (fluxY(U) is fluxX of the exchanged state with the hu and hv
+//      components exchanged back, and |v| is the exchanged |u|), so all items
+//      follow one code path. Each interface stores the flux jumps
+//      F* - F(left/lower state) and F* - F(right/upper state) and the depth
+//      jumps h* - h, which are the jumps the reference forms for each of the
+//      two elements (sgn2d.cpp:251-257, :268-275); y lines are double-buffered.
+//   C  nodes: the complete RHS of row r.
+// Every y-interface is evaluated once; U is read once per chunk, plus the
+// top faces of the row below the chunk and the x-halo nodes.
+template <int TW>
+struct Sweep {
+  static constexpr int NT = ((NPP * TW + 63) / 64) * 64;
+  static constexpr int NXI = NP * (TW + 1);   // x-interface nodes per row
+  static constexpr int NYI = NP * TW;         // y-interface nodes per line
+};
+
+template <int TW>
+__global__ void __launch_bounds__(Sweep<TW>::NT, 3)
+k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph,
+            GhostTime gt, int nStrips) {
+  constexpr int NT = Sweep<TW>::NT, NXI = Sweep<TW>::NXI, NYI = Sweep<TW>::NYI;
+  __shared__ double sU[NV][NT];
+  __shared__ double sF[NV][NT];
+  __shared__ double sG[NV][NT];
+  __shared__ double xLo[NV][NXI], xHi[NV][NXI];        // F* - F(left),  F* - F(right)
+  __shared__ double xhLo[NXI], xhHi[NXI];              // h* - h(left),  h* - h(right)
+  __shared__ double yLo[2][NV][NYI], yHi[2][NV][NYI];  // F* - F(lower), F* - F(upper)
+  __shared__ double yhLo[2][NYI], yhHi[2][NYI];
+  __shared__ double sD[NPP], sgL[NP], sgR[NP];
+
+  if (threadIdx.x < NPP) sD[threadIdx.x] = g.ops->D[threadIdx.x];
+  if (threadIdx.x < NP) {
+    sgL[threadIdx.x] = g.ops->gLp[threadIdx.x];
+    sgR[threadIdx.x] = g.ops->gRp[threadIdx.x];
+  }
+
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int chunk = tile / nStrips;
+  const int ex0 = (tile - chunk * nStrips) * TW;
+  const int nW = min(TW, g.Nx - ex0);
+  const int ey0 = chunk * g.rows;
+  const int ey1 = min(g.Ny, ey0 + g.rows);
+  const int nNodes = NPP * nW;
+  const int nX = NP * (nW + 1);
+  const int t = threadIdx.x;
+  const long long Nd = g.Nd;
+  const int Nx = g.Nx, Ny = g.Ny;
+
+  // node work item: node (a, b) of element j of the strip
+  const int j = t / NPP, loc = t - NPP * j;
+  const int b = loc / NP, a = loc - NP * b;
+  const bool isNode = t < nNodes;
+  // y-interface work item: column jj, line node aa (items NXI .. NXI + 5 nW - 1)
+  const int ky = t - NXI;
+  const bool isY = ky >= 0 && ky < NP * nW;
+  const int jj = isY ? ky / NP : 0, aa = isY ? ky - NP * jj : 0;
+
+  // y-interface line y between rows y-1 and y: lower / upper states and jumps
+  auto y_line = [&](int y, const double* SLin, const double* SRin, int buf) {
+    double SL[NV], SR[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { SL[v] = SLin[v]; SR[v] = SRin[v]; }
+    if (y == 0 && g.bc[2] != 0) wall_ghost(SR, IHV, SL);
+    if (y == Ny && g.bc[3] != 0) wall_ghost(SL, IHV, SR);
+    const double Hf = g.HfY[((long long)y * Nx + ex0 + jj) * NP + aa];
+    // exchange hu <-> hv: the y-interface as an x-interface
+    double x;
+    x = SL[IHU]; SL[IHU] = SL[IHV]; SL[IHV] = x;
+    x = SR[IHU]; SR[IHU] = SR[IHV]; SR[IHV] = x;
+    double Fs[NV], FL[NV], FR[NV];
+    interface_flux<0>(SL, SR, Hf, ph, Fs, FL, FR);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int w = (v == IHU) ? IHV : (v == IHV) ? IHU : v;
+      yLo[buf][w][ky] = Fs[v] - FL[v];
+      yHi[buf][w][ky] = Fs[v] - FR[v];
+    }
+    const double hs = 0.5 * (SL[IH] + SR[IH]);
+    yhLo[buf][ky] = hs - SL[IH];
+    yhHi[buf][ky] = hs - SR[IH];
+  };
+
+  // lower / upper face states of a y line from global memory (wrapped rows;
+  // at a wall side the state is replaced by the ghost inside y_line)
+  auto y_state_below = [&](int y, double* W) {       // top face of row y-1
+    if (y > 0) load_face(U, Nd, nidx(g, ex0 + jj, y - 1, aa, NP - 1), W);
+    else if (g.bc[2] == 0) load_face(U, Nd, nidx(g, ex0 + jj, Ny - 1, aa, NP - 1), W);
+    else for (int v = 0; v < NV; ++v) W[v] = 0.0;
+  };
+  auto y_state_above = [&](int y, double* W) {       // bottom face of row y
+    if (y < Ny) load_face(U, Nd, nidx(g, ex0 + jj, y, aa, 0), W);
+    else if (g.bc[3] == 0) load_face(U, Nd, nidx(g, ex0 + jj, 0, aa, 0), W);
+    else for (int v = 0; v < NV; ++v) W[v] = 0.0;
+  };
+
+  // prologue: the y line at the bottom of the chunk
+  if (isY) {
+    double SL[NV], SR[NV];
+    y_state_below(ey0, SL);
+    y_state_above(ey0, SR);
+    y_line(ey0, SL, SR, ey0 & 1);
+  }
+
+  // register prefetch of row ey0's node data
+  double pU[NV], pH = 0.0, pHx = 0.0, pHy = 0.0;
+  if (isNode) {
+    const long long gn = ((long long)ey0 * Nx + ex0) * NPP + t;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) pU[v] = U[(long long)v * Nd + gn];
+    pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
+  }
+
+  for (int r = ey0; r < ey1; ++r) {
+    const long long rowBase = ((long long)r * Nx + ex0) * NPP;
+    // ---- A ---------------------------------------------------------------------
+    double Yup[NV];
+    if (isY) y_state_above(r + 1, Yup);             // bottom face of row r+1
+    double S[NV], qu = 0.0, qv = 0.0;
+    if (isNode) {
+      double Un[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) Un[v] = pU[v];
+      const double Hn = pH, Hxn = pHx, Hyn = pHy;
+      if (r + 1 < ey1) {
+        const long long gn = rowBase + (long long)Nx * NPP + t;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) pU[v] = U[(long long)v * Nd + gn];
+        pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
+      }
+      const Prim q = prims(Un);
+      qu = q.u;
+      qv = q.v;
+      source(Un, q, Hn, Hxn, Hyn, ph, S);
+      double F[NV], G[NV];
+      flux_x(Un, q, Hn, ph, F);
+      flux_y(Un, q, Hn, ph, G);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        sU[v][t] = Un[v];
+        sF[v][t] = F[v];
+        sG[v][t] = G[v];
+      }
+    }
+    __syncthreads();
+
+    // ---- B ---------------------------------------------------------------------
+    if (t < nX) {
+      const int i = t / NP, bb = t - NP * i;
+      const int ex = ex0 + i;                   // left edge of column ex
+      double SL[NV], SR[NV];
+      bool ghostL = false, ghostR = false;
+      if (i < nW) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) SR[v] = sU[v][i * NPP + NP * bb] + 0.0;
+      } else if (ex < Nx) {
+        load_face(U, Nd, nidx(g, ex, r, 0, bb), SR);
+      } else if (g.bc[1] == 0) {
+        load_face(U, Nd, nidx(g, 0, r, 0, bb), SR);
+      } else {
+        ghostR = true;
+      }
+      if (i > 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) SL[v] = sU[v][(i - 1) * NPP + (NP - 1) + NP * bb] + 0.0;
+      } else if (ex > 0) {
+        load_face(U, Nd, nidx(g, ex - 1, r, NP - 1, bb), SL);
+      } else if (g.bc[0] == 0) {
+        load_face(U, Nd, nidx(g, Nx - 1, r, NP - 1, bb), SL);
+      } else {
+        ghostL = true;
+      }
+      if (ghostL) {
+        if (g.bc[0] == 1) {
+          wall_ghost(SR, IHU, SL);
+        } else {
+          const double* c = g.inflow + ((long long)r * NP + bb) * (NV * 3);
+#pragma unroll
+          for (int v = 0; v < NV; ++v)
+            SL[v] = (c[3 * v] + c[3 * v + 1] * gt.s) + c[3 * v + 2] * gt.c;
+        }
+      }
+      if (ghostR) wall_ghost(SL, IHU, SR);
+      const double Hf = g.HfX[((long long)r * (Nx + 1) + ex) * NP + bb];
+      double Fs[NV], FL[NV], FR[NV];
+      interface_flux<0>(SL, SR, Hf, ph, Fs, FL, FR);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        xLo[v][t] = Fs[v] - FL[v];
+        xHi[v][t] = Fs[v] - FR[v];
+      }
+      const double hs = 0.5 * (SL[IH] + SR[IH]);
+      xhLo[t] = hs - SL[IH];
+      xhHi[t] = hs - SR[IH];
+    } else if (isY) {
+      double SL[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) SL[v] = sU[v][jj * NPP + aa + NP * (NP - 1)] + 0.0;
+      y_line(r + 1, SL, Yup, (r + 1) & 1);
+    }
+    __syncthreads();
+
+    // ---- C: dU of row r ---------------------------------------------------------
+    if (isNode) {
+      double Da[NP], Db[NP];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) { Da[k] = sD[a * NP + k]; Db[k] = sD[b * NP + k]; }
+      const double gLa = sgL[a], gRa = sgR[a], gLb = sgL[b], gRb = sgR[b];
+      const int ixL = j * NP + b, ixR = (j + 1) * NP + b, iy = j * NP + a;
+      const int bot = r & 1, top = (r + 1) & 1;
+      const int rowx = j * NPP + NP * b, coly = j * NPP + a;
+      double dhx = 0.0, dhy = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        dhx = dhx + Da[k] * sU[IH][rowx + k];
+        dhy = dhy + Db[k] * sU[IH][coly + NP * k];
+      }
+      dhx = dhx + (xhHi[ixL] * gLa + xhLo[ixR] * gRa);
+      dhy = dhy + (yhHi[bot][iy] * gLb + yhLo[top][iy] * gRb);
+      const double hx = div_const(dhx, g.dx), hy = div_const(dhy, g.dy);
+      // the hp entries of the non-conservative products (the others are zero,
+      // and subtracting zero changes no sum)
+      const double Bx = ph.mc2 * qu * hx;
+      const double By = ph.mc2 * qv * hy;
+      const long long gn = rowBase + t;
+      // one variable per trip keeps the LDS operands of a single variable live
+#pragma unroll 1
+      for (int v = 0; v < NV; ++v) {
+        double dfx = 0.0, dgy = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          dfx = dfx + Da[k] * sF[v][rowx + k];
+          dgy = dgy + Db[k] * sG[v][coly + NP * k];
+        }
+        const double cx = xHi[v][ixL] * gLa + xLo[v][ixR] * gRa;
+        const double cy = yHi[bot][v][iy] * gLb + yLo[top][v][iy] * gRb;
+        double o = -div_const(dfx + cx, g.dx) - div_const(dgy + cy, g.dy);
+        if (v == IHP) o = o - Bx - By;
+        const double sv = (v == IH) ? S[IH] : (v == IHU) ? S[IHU] : (v == IHV) ? S[IHV]
+                        : (v == IHW) ? S[IHW] : S[IHP];
+        dU[(long long)v * Nd + gn] = o + sv;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Hx, Hy at every node (sgn2d.cpp:54-71).
 __global__ void __launch_bounds__(kThreads)
 k_bathy_grad(const double* __restrict__ H, double* __restrict__ Hx, double* __restrict__ Hy,
@@ -429,12 +687,21 @@ inline unsigned elementwise_blocks(size_t n) {
   return (unsigned)(want < 16384 ? (want ? want : 1) : 16384);
 }
 
-constexpr int kTileW = 10;   // elements per RHS tile
+constexpr int kTileW = 10;   // elements per RHS tile / sweep strip
 
 }  // namespace
 
 hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double* dU,
                       GhostTime gt, hipStream_t st) {
+  if (g.variant == 1) {
+    const int strips = (g.Nx + kTileW - 1) / kTileW;
+    const long long chunks = (g.Ny + g.rows - 1) / g.rows;
+    const long long tiles = (long long)strips * chunks;
+    if (tiles > 0x7fffffffLL || g.rows < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rhs_sweep<kTileW>, dim3((unsigned)tiles), dim3(Sweep<kTileW>::NT), 0,
+                       st, U, dU, g, ph, gt, strips);
+    return hipGetLastError();
+  }
   const int tpr = (g.Nx + kTileW - 1) / kTileW;
   const long long tiles = (long long)tpr * g.Ny;
   if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;

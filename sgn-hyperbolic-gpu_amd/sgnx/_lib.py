@@ -10,7 +10,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libsgnx.so"))
+LIB_PATH = os.environ.get("SGNX_LIB") or os.path.normpath(os.path.join(_HERE, "..", "libsgnx.so"))
 
 SGNX_OK = 0
 SGNX_EINVAL = 1
@@ -93,6 +93,9 @@ SIGNATURES = [
     ("sgnx_mass", C.c_int, [C.c_void_p, _dp]),
     ("sgnx_energy", C.c_int, [C.c_void_p, _dp]),
     ("sgnx_bathymetry", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("sgnx_set_rhs_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("sgnx_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("sgnx_rhs_timing", C.c_int, [C.c_void_p, C.POINTER(C.c_long), _dp]),
     ("sgnx_fr_operators", C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("sgnx_node_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]),
     ("sgnx_face_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

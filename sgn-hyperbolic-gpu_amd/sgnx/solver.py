@@ -266,6 +266,20 @@ class SGN2D:
     def energy(self, U=None):
         return self._diag(self.lib.sgnx_energy, U)
 
+    def set_rhs_kernel(self, variant, rows=16):
+        """0: one workgroup per 10-element row tile; 1: row sweep of `rows` rows."""
+        L.check(self.lib.sgnx_set_rhs_kernel(self.h, int(variant), int(rows)), self.h)
+
+    def enable_timing(self, on=True):
+        L.check(self.lib.sgnx_enable_timing(self.h, int(on)), self.h)
+
+    def rhs_timing(self):
+        """(launches, total device ms) of RHS launches since timing was enabled."""
+        n = C.c_long()
+        ms = C.c_double()
+        L.check(self.lib.sgnx_rhs_timing(self.h, C.byref(n), C.byref(ms)), self.h)
+        return n.value, ms.value
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.sgnx_destroy(self.h)

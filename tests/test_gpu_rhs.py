@@ -26,13 +26,26 @@ def assert_bitwise(got, want, what):
                              f"max abs diff {np.nanmax(np.abs(got - want)):.3e}")
 
 
-def run_pair(case, t=0.0, U=None):
+# RHS kernel variants checked by every parity case: (variant, rows per chunk)
+KERNELS = [(0, 16), (1, 16), (1, 3), (1, 1)]
+
+
+def run_pair(case, t=0.0, U=None, kernels=KERNELS):
+    """GPU RHS under every kernel variant; all must agree with the oracle bitwise.
+    Returns the first variant's result and the oracle's."""
     import sgnx
     U = case.U0 if U is None else U
     s = sgnx.SGN2D(**case.solver_kwargs())
     o = OL.Oracle.from_case(case)
     try:
-        return s.rhs(U, t), o.rhs(U, t)
+        want = o.rhs(U, t)
+        outs = []
+        for variant, rows in kernels:
+            s.set_rhs_kernel(variant, rows)
+            got = s.rhs(U, t)
+            assert_bitwise(got, want, f"{case.name} kernel variant {variant} rows {rows}")
+            outs.append(got)
+        return outs[0], want
     finally:
         s.close()
         o.close()
@@ -44,7 +57,7 @@ def test_rhs_gate_case_bitwise():
     assert_bitwise(got, want, "gate 8x6 RHS")
 
 
-@pytest.mark.parametrize("Nx,Ny", [(2, 2), (3, 5), (10, 3), (11, 4), (23, 7), (40, 9)])
+@pytest.mark.parametrize("Nx,Ny", [(2, 2), (3, 5), (10, 3), (11, 4), (23, 7), (40, 9), (21, 33)])
 @pytest.mark.parametrize("sides", ["periodic", "walls_y", "walls"])
 def test_rhs_random_bitwise(Nx, Ny, sides):
     from sgnx import cases
@@ -117,21 +130,22 @@ def test_bathymetry_gradients_bitwise():
 
 
 def test_rhs_device_pointers_and_determinism():
-    """sgnx_rhs on device pointers; two evaluations give identical bits."""
-    import torch
+    """sgnx_rhs on caller-owned device buffers; repeated evaluations give
+    identical bits."""
     import sgnx
-    from sgnx import cases
+    from sgnx import cases, hip
     case = cases.gate(Nx=17, Ny=9)
     s = sgnx.SGN2D(**case.solver_kwargs())
-    U = torch.from_numpy(case.U0).to("cuda:0")
-    d1 = torch.empty_like(U)
-    d2 = torch.empty_like(U)
-    torch.cuda.synchronize()
-    s.rhs_device(U.data_ptr(), d1.data_ptr())
-    s.rhs_device(U.data_ptr(), d2.data_ptr())
-    torch.cuda.synchronize()
-    a, b = d1.cpu().numpy(), d2.cpu().numpy()
+    U = hip.DeviceArray.from_host(case.U0)
+    d1 = hip.DeviceArray(U.n)
+    d2 = hip.DeviceArray(U.n)
+    s.rhs_device(U.address, d1.address)
+    s.rhs_device(U.address, d2.address)
+    hip.synchronize()
+    a, b = d1.download(), d2.download()
     s.close()
     assert_bitwise(a, b, "run-to-run")
     want = OL.Oracle.from_case(case).rhs(case.U0)
     assert_bitwise(a, want, "device-pointer RHS")
+    for d in (U, d1, d2):
+        d.free()
