@@ -78,6 +78,11 @@ def profile_rhs(args):
     for _ in range(3):
         b.copy_from(a)
     hip.synchronize()
+    if args.profile_steps:
+        s.upload(case.U0)
+        dt0 = s.cflTimeStep(None, 0.5)
+        st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=args.profile_steps))
+        log(f"[bench] profile: {st.accepted} adaptive steps ({st.rhsEvals} RHS)")
     log(f"[bench] profile: {args.profile_rhs} RHS launches of {case.nd} nodes; "
         f"3 device copies of {args.copy_mib} MiB")
     s.close()
@@ -130,6 +135,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-rhs", type=int, default=0)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="with --profile-rhs: also run this many adaptive steps")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -160,7 +167,7 @@ def main():
     t0 = time.perf_counter()
     st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=args.steps, **opts))
     wall = time.perf_counter() - t0                  # integrateAdaptive ends with a sync
-    launches, rhs_ms = s.rhs_timing()
+    timing = s.rhs_timing()
     s.enable_timing(False)
     if st.aborted or st.accepted != args.steps:
         log(f"[bench] run did not complete: {st.as_dict()}")
@@ -169,8 +176,16 @@ def main():
     nd = case.nd
     dof = 5.0 * nd
     value = dof * st.rhsEvals / wall
+    # algorithmic bytes per node of an RHS launch by stage input: the RHS
+    # itself (104 B) plus the stage operands read and results written in the
+    # same launch (u + a*k: one more state read; candidate: three more state
+    # reads and the uNew write)
+    alg = [ALG_BYTES_PER_NODE, ALG_BYTES_PER_NODE + 40, ALG_BYTES_PER_NODE + 160]
+    launches = sum(n for n, _ in timing)
+    rhs_ms = sum(ms for _, ms in timing)
     mean_rhs_s = rhs_ms / launches / 1e3
-    achieved = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
+    achieved = sum(alg[m] * nd * timing[m][0] for m in range(3)) / (rhs_ms / 1e3) / 1e9
+    achieved_104 = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "rhs_traffic.json")
     if os.path.exists(tpath):
@@ -216,10 +231,15 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_rhs (fused FR/SGN right-hand side)",
+            "kernel": "k_rhs_sweep (FR/SGN right-hand side, stage input fused)",
             "mean_launch_ms": mean_rhs_s * 1e3,
             "launches": launches,
-            "alg_bytes_per_launch": ALG_BYTES_PER_NODE * nd,
+            "launches_by_stage": {"plain": timing[0][0], "u+a*k": timing[1][0],
+                                  "candidate": timing[2][0]},
+            "mean_ms_by_stage": {k: (timing[m][1] / timing[m][0] if timing[m][0] else None)
+                                 for m, k in enumerate(("plain", "u+a*k", "candidate"))},
+            "alg_bytes_per_node_by_stage": {"plain": alg[0], "u+a*k": alg[1], "candidate": alg[2]},
+            "achieved_rhs_only_104B": achieved_104,
         },
         "cpu_baseline": cpu,
     }

@@ -47,10 +47,12 @@ struct sgnx_solver {
   double inflowOmega = 0.0;
   bool hasInflow = false;
   std::string err;
-  // launch timing: event pairs around RHS launches while enabled
+  // launch timing: event pairs around RHS launches while enabled, by stage mode
   bool timing = false;
   std::vector<hipEvent_t> events;   // 2 per recorded launch, reused
+  std::vector<int> timedMode;       // stage mode of each recorded launch
   long timed = 0;
+  long long partCap = 0;            // doubles in d_part
 
   size_t n() const { return (size_t)NV * (size_t)g.Nd; }
 };
@@ -93,22 +95,34 @@ GhostTime ghost_time(const sgnx_solver* s, double t) {
   return gt;
 }
 
-int rhs_dev(sgnx_solver* s, const double* U, double* dU, double t) {
-  if (!s->timing) {
-    HIPCHK(s, launch_rhs(s->g, s->ph, U, dU, ghost_time(s, t), s->stream));
-    return SGNX_OK;
+// One RHS launch: plain input (mode kPlain, either kernel variant) or a fused
+// stage input (row sweep only), bracketed by timing events when enabled.
+int rhs_launch(sgnx_solver* s, int mode, const Stage& sa, double* dU, double t) {
+  const GhostTime gt = ghost_time(s, t);
+  if (s->timing) {
+    const size_t need = 2 * (size_t)(s->timed + 1);
+    while (s->events.size() < need) {
+      hipEvent_t e;
+      HIPCHK(s, hipEventCreate(&e));
+      s->events.push_back(e);
+    }
+    if (s->timedMode.size() < (size_t)(s->timed + 1)) s->timedMode.resize(s->timed + 1);
+    s->timedMode[s->timed] = mode;
+    HIPCHK(s, hipEventRecord(s->events[2 * s->timed], s->stream));
   }
-  const size_t need = 2 * (size_t)(s->timed + 1);
-  while (s->events.size() < need) {
-    hipEvent_t e;
-    HIPCHK(s, hipEventCreate(&e));
-    s->events.push_back(e);
+  if (mode == kPlain) HIPCHK(s, launch_rhs(s->g, s->ph, sa.u, dU, gt, s->stream));
+  else HIPCHK(s, launch_rhs_stage(s->g, s->ph, mode, sa, dU, gt, s->stream));
+  if (s->timing) {
+    HIPCHK(s, hipEventRecord(s->events[2 * s->timed + 1], s->stream));
+    ++s->timed;
   }
-  HIPCHK(s, hipEventRecord(s->events[2 * s->timed], s->stream));
-  HIPCHK(s, launch_rhs(s->g, s->ph, U, dU, ghost_time(s, t), s->stream));
-  HIPCHK(s, hipEventRecord(s->events[2 * s->timed + 1], s->stream));
-  ++s->timed;
   return SGNX_OK;
+}
+
+int rhs_dev(sgnx_solver* s, const double* U, double* dU, double t) {
+  Stage sa{};
+  sa.u = U;
+  return rhs_launch(s, kPlain, sa, dU, t);
 }
 
 int sync(sgnx_solver* s) {
@@ -149,6 +163,9 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
     return fail(nullptr, SGNX_EINVAL, "inflow side without an inflow table");
   if (d->Hnod && (!d->HfaceX || !d->HfaceY))
     return fail(nullptr, SGNX_EINVAL, "face still depths required with a nodal still depth");
+  // node offsets within one variable's array are 32-bit byte offsets in the kernels
+  if ((long long)d->Nx * d->Ny * NPP > (1LL << 28))
+    return fail(nullptr, SGNX_EUNSUPPORTED, "more than 2^28 nodes per variable");
 
   auto* s = new (std::nothrow) sgnx_solver();
   if (!s) return fail(nullptr, SGNX_ENOMEM, "host allocation failed");
@@ -166,6 +183,8 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   g.rows = 16;
   if (const char* v = std::getenv("SGNX_RHS_VARIANT")) g.variant = std::atoi(v) == 0 ? 0 : 1;
   if (const char* r = std::getenv("SGNX_SWEEP_ROWS")) g.rows = std::max(1, std::atoi(r));
+  g.waves3 = 0x3;   // plain and u + a*k at 3 waves/SIMD, the candidate at 2
+  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) g.waves3 = std::atoi(w) & 0x7;
   for (int k = 0; k < 4; ++k) g.bc[k] = d->bc[k];
   Phys& ph = s->ph;
   ph.g = d->par.g; ph.c = d->par.c; ph.gamma = d->par.gamma; ph.nManning = d->par.nManning;
@@ -235,7 +254,13 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   for (double** p : {&s->d_U, &s->d_k1, &s->d_k2, &s->d_k3, &s->d_k4, &s->d_tmp, &s->d_uNew})
     CK(hipMalloc((void**)p, nst * sizeof(double)));
   CK(hipMemset(s->d_U, 0, nst * sizeof(double)));
-  CK(hipMalloc((void**)&s->d_part, 3 * kReduceBlocks * sizeof(double)));
+  {
+    // partial triples: kReduceBlocks for the elementwise reductions, one per
+    // workgroup of the fused candidate stage (at most strips * Ny)
+    const long long strips = (d->Nx + 9) / 10;
+    s->partCap = 3 * std::max<long long>(kReduceBlocks, strips * d->Ny);
+    CK(hipMalloc((void**)&s->d_part, (size_t)s->partCap * sizeof(double)));
+  }
   CK(hipMalloc((void**)&s->d_out, 2 * sizeof(double)));
   CK(hipHostMalloc((void**)&s->h_out, 2 * sizeof(double), hipHostMallocDefault));
 #undef CK
@@ -322,15 +347,38 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
   while (t < tEnd - 1e-15 * std::max(1.0, std::abs(tEnd))) {
     if (o->max_accepted > 0 && S.accepted >= o->max_accepted) break;
     if (t + dt > tEnd) dt = tEnd - t;
-    HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
-    if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
-    HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.75 * dt, s->d_k2, n, s->stream));
-    if ((rc = rhs_dev(s, s->d_tmp, s->d_k3, t + 0.75 * dt))) return rc;
-    HIPCHK(s, launch_bs3_update(s->d_uNew, s->d_U, dt, s->d_k1, s->d_k2, s->d_k3, n, s->stream));
-    if ((rc = rhs_dev(s, s->d_uNew, s->d_k4, t + dt))) return rc;
+    if (s->g.variant == 1) {
+      // stage inputs, the candidate and its embedded error fused into the RHS
+      Stage sa{};
+      sa.u = s->d_U;
+      sa.k1 = s->d_k1;
+      sa.a = 0.5 * dt;
+      if ((rc = rhs_launch(s, kAxpy, sa, s->d_k2, t + 0.5 * dt))) return rc;
+      sa.k1 = s->d_k2;
+      sa.a = 0.75 * dt;
+      if ((rc = rhs_launch(s, kAxpy, sa, s->d_k3, t + 0.75 * dt))) return rc;
+      sa.k1 = s->d_k1;
+      sa.k2 = s->d_k2;
+      sa.k3 = s->d_k3;
+      sa.a = dt;
+      sa.uNew = s->d_uNew;
+      sa.atol = o->atol;
+      sa.rtol = o->rtol;
+      sa.part = s->d_part;
+      if ((rc = rhs_launch(s, kBs3, sa, s->d_k4, t + dt))) return rc;
+      HIPCHK(s, launch_finish(s->d_part, sweep_tiles(s->g), s->d_out, s->stream));
+    } else {
+      HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
+      if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
+      HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.75 * dt, s->d_k2, n, s->stream));
+      if ((rc = rhs_dev(s, s->d_tmp, s->d_k3, t + 0.75 * dt))) return rc;
+      HIPCHK(s, launch_bs3_update(s->d_uNew, s->d_U, dt, s->d_k1, s->d_k2, s->d_k3, n,
+                                  s->stream));
+      if ((rc = rhs_dev(s, s->d_uNew, s->d_k4, t + dt))) return rc;
+      HIPCHK(s, launch_err_norm(s->d_U, s->d_uNew, s->d_k1, s->d_k2, s->d_k3, s->d_k4, dt,
+                                o->atol, o->rtol, n, s->d_part, s->d_out, s->stream));
+    }
     S.rhsEvals += 3;
-    HIPCHK(s, launch_err_norm(s->d_U, s->d_uNew, s->d_k1, s->d_k2, s->d_k3, s->d_k4, dt, o->atol,
-                              o->rtol, n, s->d_part, s->d_out, s->stream));
     HIPCHK(s, hipMemcpyAsync(s->h_out, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost,
                              s->stream));
     if ((rc = sync(s))) return rc;
@@ -399,11 +447,34 @@ int sgnx_bs3_step(sgnx_solver* s, double t, double dt) {
   const size_t n = s->n();
   int rc;
   if ((rc = rhs_dev(s, s->d_U, s->d_k1, t))) return rc;
-  HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
-  if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
-  HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.75 * dt, s->d_k2, n, s->stream));
-  if ((rc = rhs_dev(s, s->d_tmp, s->d_k3, t + 0.75 * dt))) return rc;
-  HIPCHK(s, launch_bs3_update(s->d_uNew, s->d_U, dt, s->d_k1, s->d_k2, s->d_k3, n, s->stream));
+  if (s->g.variant == 1) {
+    // the fused stage kernels of the adaptive loop (the candidate launch also
+    // evaluates k4 and the error, which this fixed step does not use)
+    Stage sa{};
+    sa.u = s->d_U;
+    sa.k1 = s->d_k1;
+    sa.a = 0.5 * dt;
+    if ((rc = rhs_launch(s, kAxpy, sa, s->d_k2, t + 0.5 * dt))) return rc;
+    sa.k1 = s->d_k2;
+    sa.a = 0.75 * dt;
+    if ((rc = rhs_launch(s, kAxpy, sa, s->d_k3, t + 0.75 * dt))) return rc;
+    sa.k1 = s->d_k1;
+    sa.k2 = s->d_k2;
+    sa.k3 = s->d_k3;
+    sa.a = dt;
+    sa.uNew = s->d_uNew;
+    sa.atol = 1.0;
+    sa.rtol = 0.0;
+    sa.part = s->d_part;
+    if ((rc = rhs_launch(s, kBs3, sa, s->d_k4, t + dt))) return rc;
+  } else {
+    HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
+    if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
+    HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.75 * dt, s->d_k2, n, s->stream));
+    if ((rc = rhs_dev(s, s->d_tmp, s->d_k3, t + 0.75 * dt))) return rc;
+    HIPCHK(s, launch_bs3_update(s->d_uNew, s->d_U, dt, s->d_k1, s->d_k2, s->d_k3, n,
+                                s->stream));
+  }
   std::swap(s->d_U, s->d_uNew);
   return sync(s);
 }
@@ -465,14 +536,14 @@ int sgnx_rhs_timing(sgnx_solver* s, long* launches, double* total_ms) {
   if (!s || !launches || !total_ms) return SGNX_EINVAL;
   int rc = sync(s);
   if (rc) return rc;
-  double sum = 0.0;
+  for (int m = 0; m < 3; ++m) { launches[m] = 0; total_ms[m] = 0.0; }
   for (long i = 0; i < s->timed; ++i) {
     float ms = 0.0f;
     HIPCHK(s, hipEventElapsedTime(&ms, s->events[2 * i], s->events[2 * i + 1]));
-    sum += ms;
+    const int m = s->timedMode[i];
+    ++launches[m];
+    total_ms[m] += ms;
   }
-  *launches = s->timed;
-  *total_ms = sum;
   s->timed = 0;
   return SGNX_OK;
 }

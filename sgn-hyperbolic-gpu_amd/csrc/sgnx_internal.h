@@ -41,17 +41,46 @@ struct Geometry {
   ConstDiv dx, dy;           // division by Jx, Jy
   int variant;               // RHS kernel: 0 element-row tiles, 1 row sweep
   int rows;                  // row sweep: element rows per workgroup
+  int waves3;                // row sweep: bit m set = stage mode m built for 3 waves/SIMD
 };
 
 // Inflow ghost time factors for one RHS evaluation: sin(w t), cos(w t),
 // evaluated on the host with the C library, as the oracle does.
 struct GhostTime { double s, c; };
 
+// Stage input formed while loading (row-sweep kernel), with the arithmetic of
+// src/solver/rk.cpp:100-108:
+//   kPlain   U
+//   kAxpy    u + a*k1                                (stages 2 and 3)
+//   kBs3     u + a*((2/9 k1 + 1/3 k2) + 4/9 k3)      (candidate uNew, a = dt);
+//            the launch also writes uNew and the scaled embedded error of
+//            rk.cpp:118-131 with its own output as k4, one (sum, carry, max)
+//            triple per workgroup in `part`
+enum StageMode { kPlain = 0, kAxpy = 1, kBs3 = 2 };
+struct Stage {
+  const double* u;
+  const double* k1;
+  const double* k2;
+  const double* k3;
+  double a;
+  double* uNew;
+  double atol, rtol;
+  double* part;
+};
+
 // ---- launchers (sgnx_kernels.hip); all asynchronous on `st` ----------------
 
 // dU = R(U)
 hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double* dU,
                       GhostTime gt, hipStream_t st);
+// dU = R(stage input) with the row-sweep kernel; mode kBs3 also writes
+// sa.uNew and sa.part (3 doubles per workgroup, folded by launch_finish).
+hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const Stage& sa,
+                            double* dU, GhostTime gt, hipStream_t st);
+// out[0] = sum, out[1] = max of nb (sum, carry, max) partial triples, fixed order
+hipError_t launch_finish(const double* part, long long nb, double* out, hipStream_t st);
+// workgroups of one row-sweep launch (sizes the partial buffer)
+long long sweep_tiles(const Geometry& g);
 // Hx, Hy = (D . Hnod) / J per element line (sgn2d.cpp:54-71)
 hipError_t launch_bathy_grad(const Geometry& g, const double* Hnod, double* Hx, double* Hy,
                              hipStream_t st);

@@ -63,6 +63,16 @@ __device__ __forceinline__ void load_face(const double* __restrict__ U, long lon
   for (int v = 0; v < NV; ++v) W[v] = U[(long long)v * Nd + node] + 0.0;
 }
 
+// Compensated accumulation (Knuth's TwoSum): s + c carries the running sum
+// with the rounding error of every addition kept in c.
+__device__ __forceinline__ void acc_add(double& s, double& c, double x) {
+  const double t = s + x;
+  const double bp = t - s;
+  const double e = (s - (t - bp)) + (x - bp);
+  s = t;
+  c = c + e;
+}
+
 // Ghost state across a non-periodic side.
 __device__ __forceinline__ void wall_ghost(const double* In, int normal, double* G) {
 #pragma unroll
@@ -256,10 +266,9 @@ k_rhs(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph
 //
 // A workgroup owns a strip of TW element columns and a chunk of `rows`
 // element rows and walks up the chunk one element row r per iteration:
-//   A  the row's nodes (prefetched into registers during the previous
-//      iteration): primitives, nodal F and G -> LDS, next row's prefetch
-//      issued; the y-interface items issue the loads of the next row's bottom
-//      face nodes
+//   A  the row's nodes (loads issued during the previous iteration): stage
+//      input, primitives, nodal F and G, source -> LDS / registers; the
+//      y-interface items issue the loads of the next row's bottom-face nodes
 //   B  interface items: the TW+1 x-interface lines of row r, and the
 //      y-interface line r+1 on top of the row (the line at its bottom was
 //      done in the previous iteration, or before the loop). A y-interface is
@@ -270,9 +279,10 @@ k_rhs(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph
 //      F* - F(left/lower state) and F* - F(right/upper state) and the depth
 //      jumps h* - h, which are the jumps the reference forms for each of the
 //      two elements (sgn2d.cpp:251-257, :268-275); y lines are double-buffered.
-//   C  nodes: the complete RHS of row r.
-// Every y-interface is evaluated once; U is read once per chunk, plus the
-// top faces of the row below the chunk and the x-halo nodes.
+//   C  nodes: the loads of the next row are issued, then the complete RHS of
+//      row r (and, in the candidate stage, its scaled embedded error).
+// Every y-interface is evaluated once; the stage input is read once per chunk,
+// plus the top faces of the row below the chunk and the x-halo nodes.
 template <int TW>
 struct Sweep {
   static constexpr int NT = ((NPP * TW + 63) / 64) * 64;
@@ -280,10 +290,71 @@ struct Sweep {
   static constexpr int NYI = NP * TW;         // y-interface nodes per line
 };
 
-template <int TW>
-__global__ void __launch_bounds__(Sweep<TW>::NT, 3)
+// Raw operands of the stage input at one node (5 variables each).
+template <int MODE>
+struct StageRaw {
+  double u[NV], k1[MODE >= 1 ? NV : 1], k2[MODE == 2 ? NV : 1], k3[MODE == 2 ? NV : 1];
+};
+
+// Uniform per-variable base pointers: a node's value is base[v][node] with a
+// 32-bit node offset, so loads use a scalar base plus a 32-bit lane offset
+// instead of one 64-bit lane address per operand.
+struct VarPtrs {
+  const double* p[NV];
+};
+__device__ __forceinline__ VarPtrs var_ptrs(const double* base, long long Nd) {
+  VarPtrs r;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) r.p[v] = base ? base + (long long)v * Nd : nullptr;
+  return r;
+}
+
+struct StagePtrs {
+  VarPtrs u, k1, k2, k3;
+};
+
+template <int MODE>
+__device__ __forceinline__ void stage_load(const StagePtrs& P, unsigned node, StageRaw<MODE>& R) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    R.u[v] = P.u.p[v][node];
+    if (MODE >= kAxpy) R.k1[v] = P.k1.p[v][node];
+    if (MODE == kBs3) {
+      R.k2[v] = P.k2.p[v][node];
+      R.k3[v] = P.k3.p[v][node];
+    }
+  }
+}
+
+// rk.cpp:100-108: the stage input from its operands
+template <int MODE>
+__device__ __forceinline__ void stage_combine(const StageRaw<MODE>& R, double a, double* W) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (MODE == kPlain) W[v] = R.u[v];
+    else if (MODE == kAxpy) W[v] = R.u[v] + a * R.k1[v];
+    else W[v] = R.u[v] + a * (2.0 / 9.0 * R.k1[v] + 1.0 / 3.0 * R.k2[v] + 4.0 / 9.0 * R.k3[v]);
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void stage_face(const StagePtrs& P, double a, unsigned node,
+                                           double* W) {
+  StageRaw<MODE> R;
+  stage_load<MODE>(P, node, R);
+  stage_combine<MODE>(R, a, W);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) W[v] = W[v] + 0.0;
+}
+
+__device__ __forceinline__ unsigned node_of(int Nx, int ex, int ey, int a, int b) {
+  return ((unsigned)ey * (unsigned)Nx + (unsigned)ex) * NPP + a + NP * b;
+}
+
+template <int TW, int MODE, int WAVES>
+__global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
 k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph,
-            GhostTime gt, int nStrips) {
+            GhostTime gt, Stage sa, int nStrips) {
   constexpr int NT = Sweep<TW>::NT, NXI = Sweep<TW>::NXI, NYI = Sweep<TW>::NYI;
   __shared__ double sU[NV][NT];
   __shared__ double sF[NV][NT];
@@ -311,6 +382,11 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   const int t = threadIdx.x;
   const long long Nd = g.Nd;
   const int Nx = g.Nx, Ny = g.Ny;
+  StagePtrs P;
+  P.u = var_ptrs(MODE == kPlain ? U : sa.u, Nd);
+  P.k1 = var_ptrs(sa.k1, Nd);
+  P.k2 = var_ptrs(sa.k2, Nd);
+  P.k3 = var_ptrs(sa.k3, Nd);
 
   // node work item: node (a, b) of element j of the strip
   const int j = t / NPP, loc = t - NPP * j;
@@ -321,7 +397,7 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   const bool isY = ky >= 0 && ky < NP * nW;
   const int jj = isY ? ky / NP : 0, aa = isY ? ky - NP * jj : 0;
 
-  // y-interface line y between rows y-1 and y: lower / upper states and jumps
+  // y-interface line y between rows y-1 and y: jumps into buffer `buf`
   auto y_line = [&](int y, const double* SLin, const double* SRin, int buf) {
     double SL[NV], SR[NV];
 #pragma unroll
@@ -349,13 +425,13 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   // lower / upper face states of a y line from global memory (wrapped rows;
   // at a wall side the state is replaced by the ghost inside y_line)
   auto y_state_below = [&](int y, double* W) {       // top face of row y-1
-    if (y > 0) load_face(U, Nd, nidx(g, ex0 + jj, y - 1, aa, NP - 1), W);
-    else if (g.bc[2] == 0) load_face(U, Nd, nidx(g, ex0 + jj, Ny - 1, aa, NP - 1), W);
+    if (y > 0) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, y - 1, aa, NP - 1), W);
+    else if (g.bc[2] == 0) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, Ny - 1, aa, NP - 1), W);
     else for (int v = 0; v < NV; ++v) W[v] = 0.0;
   };
   auto y_state_above = [&](int y, double* W) {       // bottom face of row y
-    if (y < Ny) load_face(U, Nd, nidx(g, ex0 + jj, y, aa, 0), W);
-    else if (g.bc[3] == 0) load_face(U, Nd, nidx(g, ex0 + jj, 0, aa, 0), W);
+    if (y < Ny) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, y, aa, 0), W);
+    else if (g.bc[3] == 0) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, 0, aa, 0), W);
     else for (int v = 0; v < NV; ++v) W[v] = 0.0;
   };
 
@@ -367,31 +443,28 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     y_line(ey0, SL, SR, ey0 & 1);
   }
 
-  // register prefetch of row ey0's node data
-  double pU[NV], pH = 0.0, pHx = 0.0, pHy = 0.0;
+  // loads of row ey0's node data
+  StageRaw<MODE> pR;
+  double pH = 0.0, pHx = 0.0, pHy = 0.0;
   if (isNode) {
-    const long long gn = ((long long)ey0 * Nx + ex0) * NPP + t;
-#pragma unroll
-    for (int v = 0; v < NV; ++v) pU[v] = U[(long long)v * Nd + gn];
+    const unsigned gn = node_of(Nx, ex0, ey0, 0, 0) + t;
+    stage_load<MODE>(P, gn, pR);
     pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
   }
+  // scaled embedded error of this work-item's nodes (candidate stage)
+  double eS = 0.0, eC = 0.0, eM = 0.0;
 
   for (int r = ey0; r < ey1; ++r) {
-    const long long rowBase = ((long long)r * Nx + ex0) * NPP;
+    const unsigned rowBase = node_of(Nx, ex0, r, 0, 0);
     // ---- A ---------------------------------------------------------------------
-    double Yup[NV];
-    if (isY) y_state_above(r + 1, Yup);             // bottom face of row r+1
     double S[NV], qu = 0.0, qv = 0.0;
     if (isNode) {
       double Un[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) Un[v] = pU[v];
+      stage_combine<MODE>(pR, sa.a, Un);
       const double Hn = pH, Hxn = pHx, Hyn = pHy;
-      if (r + 1 < ey1) {
-        const long long gn = rowBase + (long long)Nx * NPP + t;
+      if (MODE == kBs3) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) pU[v] = U[(long long)v * Nd + gn];
-        pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
+        for (int v = 0; v < NV; ++v) (sa.uNew + (long long)v * Nd)[rowBase + t] = Un[v];
       }
       const Prim q = prims(Un);
       qu = q.u;
@@ -419,9 +492,9 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
 #pragma unroll
         for (int v = 0; v < NV; ++v) SR[v] = sU[v][i * NPP + NP * bb] + 0.0;
       } else if (ex < Nx) {
-        load_face(U, Nd, nidx(g, ex, r, 0, bb), SR);
+        stage_face<MODE>(P, sa.a, node_of(Nx, ex, r, 0, bb), SR);
       } else if (g.bc[1] == 0) {
-        load_face(U, Nd, nidx(g, 0, r, 0, bb), SR);
+        stage_face<MODE>(P, sa.a, node_of(Nx, 0, r, 0, bb), SR);
       } else {
         ghostR = true;
       }
@@ -429,9 +502,9 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
 #pragma unroll
         for (int v = 0; v < NV; ++v) SL[v] = sU[v][(i - 1) * NPP + (NP - 1) + NP * bb] + 0.0;
       } else if (ex > 0) {
-        load_face(U, Nd, nidx(g, ex - 1, r, NP - 1, bb), SL);
+        stage_face<MODE>(P, sa.a, node_of(Nx, ex - 1, r, NP - 1, bb), SL);
       } else if (g.bc[0] == 0) {
-        load_face(U, Nd, nidx(g, Nx - 1, r, NP - 1, bb), SL);
+        stage_face<MODE>(P, sa.a, node_of(Nx, Nx - 1, r, NP - 1, bb), SL);
       } else {
         ghostL = true;
       }
@@ -458,15 +531,27 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
       xhLo[t] = hs - SL[IH];
       xhHi[t] = hs - SR[IH];
     } else if (isY) {
-      double SL[NV];
+      double SL[NV], SU[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) SL[v] = sU[v][jj * NPP + aa + NP * (NP - 1)] + 0.0;
-      y_line(r + 1, SL, Yup, (r + 1) & 1);
+      y_state_above(r + 1, SU);                     // bottom face of row r+1
+      y_line(r + 1, SL, SU, (r + 1) & 1);
     }
     __syncthreads();
 
     // ---- C: dU of row r ---------------------------------------------------------
+    // loads of the next row, consumed in A: issued before the lift when the
+    // stage has one or two operands, after it for the four-operand candidate
+    // (whose 20 values would otherwise stay live across the whole phase)
+    auto prefetch_next = [&]() {
+      if (r + 1 < ey1) {
+        const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
+        stage_load<MODE>(P, gn, pR);
+        pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
+      }
+    };
     if (isNode) {
+      if (MODE != kBs3) prefetch_next();
       double Da[NP], Db[NP];
 #pragma unroll
       for (int k = 0; k < NP; ++k) { Da[k] = sD[a * NP + k]; Db[k] = sD[b * NP + k]; }
@@ -487,7 +572,7 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
       // and subtracting zero changes no sum)
       const double Bx = ph.mc2 * qu * hx;
       const double By = ph.mc2 * qv * hy;
-      const long long gn = rowBase + t;
+      const unsigned gn = rowBase + t;
       // one variable per trip keeps the LDS operands of a single variable live
 #pragma unroll 1
       for (int v = 0; v < NV; ++v) {
@@ -503,10 +588,54 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
         if (v == IHP) o = o - Bx - By;
         const double sv = (v == IH) ? S[IH] : (v == IHU) ? S[IHU] : (v == IHV) ? S[IHV]
                         : (v == IHW) ? S[IHW] : S[IHP];
-        dU[(long long)v * Nd + gn] = o + sv;
+        const double k4 = o + sv;
+        // per-variable pointers formed here (a loop-indexed pointer array
+        // would live in scratch memory)
+        const long long vo = (long long)v * Nd;
+        (dU + vo)[gn] = k4;
+        if (MODE == kBs3) {
+          // rk.cpp:118-131 for this entry
+          const double u0 = (sa.u + vo)[gn], un = sU[v][t];
+          const double uEmb = u0 + sa.a * (7.0 / 24.0 * (sa.k1 + vo)[gn] +
+                                           0.25 * (sa.k2 + vo)[gn] +
+                                           1.0 / 3.0 * (sa.k3 + vo)[gn] + 0.125 * k4);
+          const double e = un - uEmb;
+          const double au = fabs(u0), an = fabs(un);
+          const double r_ = e / (sa.atol + sa.rtol * ((au < an) ? an : au));
+          acc_add(eS, eC, r_ * r_);
+          const double ar = fabs(r_);
+          eM = (eM < ar) ? ar : eM;
+        }
       }
+      if (MODE == kBs3) prefetch_next();
     }
     __syncthreads();
+  }
+
+  if (MODE == kBs3) {
+    // fixed-order reduction over the workgroup (LDS reused)
+    double* ss = &sF[0][0];
+    double* sc = &sF[1][0];
+    double* sm = &sF[2][0];
+    ss[t] = eS;
+    sc[t] = eC;
+    sm[t] = eM;
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+      if (t < off) {
+        double s = ss[t], c = sc[t];
+        acc_add(s, c, ss[t + off]);
+        ss[t] = s;
+        sc[t] = c + sc[t + off];
+        sm[t] = fmax(sm[t], sm[t + off]);
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      sa.part[3 * tile] = ss[0];
+      sa.part[3 * tile + 1] = sc[0];
+      sa.part[3 * tile + 2] = sm[0];
+    }
   }
 }
 
@@ -557,16 +686,6 @@ k_rk4_update(double* __restrict__ u, double dt, const double* __restrict__ k1,
   const double d6 = dt / 6.0;
   for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
     u[i] += d6 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
-}
-
-// Compensated accumulation (Knuth's TwoSum): s + c carries the running sum
-// with the rounding error of every addition kept in c.
-__device__ __forceinline__ void acc_add(double& s, double& c, double x) {
-  const double t = s + x;
-  const double bp = t - s;
-  const double e = (s - (t - bp)) + (x - bp);
-  s = t;
-  c = c + e;
 }
 
 // Fixed-order block reduction of (s, c, m) triples held in LDS.
@@ -691,16 +810,41 @@ constexpr int kTileW = 10;   // elements per RHS tile / sweep strip
 
 }  // namespace
 
+long long sweep_tiles(const Geometry& g) {
+  const long long strips = (g.Nx + kTileW - 1) / kTileW;
+  const long long chunks = (g.Ny + g.rows - 1) / g.rows;
+  return strips * chunks;
+}
+
+hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const Stage& sa,
+                            double* dU, GhostTime gt, hipStream_t st) {
+  const int strips = (g.Nx + kTileW - 1) / kTileW;
+  const long long tiles = sweep_tiles(g);
+  if (tiles > 0x7fffffffLL || g.rows < 1) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)tiles), block(Sweep<kTileW>::NT);
+  // waves per SIMD the register allocation targets, per stage mode (bit m of
+  // g.waves3 set: 3 waves, else 2)
+  const bool w3 = (g.waves3 >> mode) & 1;
+#define SGNX_SWEEP(M, W) \
+  hipLaunchKernelGGL((k_rhs_sweep<kTileW, M, W>), grid, block, 0, st, sa.u, dU, g, ph, gt, sa, strips)
+  if (mode == kPlain) { if (w3) SGNX_SWEEP(kPlain, 3); else SGNX_SWEEP(kPlain, 2); }
+  else if (mode == kAxpy) { if (w3) SGNX_SWEEP(kAxpy, 3); else SGNX_SWEEP(kAxpy, 2); }
+  else { if (w3) SGNX_SWEEP(kBs3, 3); else SGNX_SWEEP(kBs3, 2); }
+#undef SGNX_SWEEP
+  return hipGetLastError();
+}
+
+hipError_t launch_finish(const double* part, long long nb, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, st, part, (int)nb, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double* dU,
                       GhostTime gt, hipStream_t st) {
   if (g.variant == 1) {
-    const int strips = (g.Nx + kTileW - 1) / kTileW;
-    const long long chunks = (g.Ny + g.rows - 1) / g.rows;
-    const long long tiles = (long long)strips * chunks;
-    if (tiles > 0x7fffffffLL || g.rows < 1) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_rhs_sweep<kTileW>, dim3((unsigned)tiles), dim3(Sweep<kTileW>::NT), 0,
-                       st, U, dU, g, ph, gt, strips);
-    return hipGetLastError();
+    Stage sa{};
+    sa.u = U;
+    return launch_rhs_stage(g, ph, kPlain, sa, dU, gt, st);
   }
   const int tpr = (g.Nx + kTileW - 1) / kTileW;
   const long long tiles = (long long)tpr * g.Ny;

@@ -274,11 +274,12 @@ class SGN2D:
         L.check(self.lib.sgnx_enable_timing(self.h, int(on)), self.h)
 
     def rhs_timing(self):
-        """(launches, total device ms) of RHS launches since timing was enabled."""
-        n = C.c_long()
-        ms = C.c_double()
-        L.check(self.lib.sgnx_rhs_timing(self.h, C.byref(n), C.byref(ms)), self.h)
-        return n.value, ms.value
+        """[(launches, total device ms)] of RHS launches since timing was enabled,
+        by stage input: plain state, u + a*k, BS3 candidate."""
+        n = (C.c_long * 3)()
+        ms = (C.c_double * 3)()
+        L.check(self.lib.sgnx_rhs_timing(self.h, n, ms), self.h)
+        return [(n[m], ms[m]) for m in range(3)]
 
     def close(self):
         if getattr(self, "h", None):

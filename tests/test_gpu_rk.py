@@ -18,11 +18,15 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_fixed_steps_bitwise():
+@pytest.mark.parametrize("variant,rows", [(0, 16), (1, 16), (1, 2)])
+def test_fixed_steps_bitwise(variant, rows):
+    """Fixed RK4 and BS3 steps; with the row-sweep variant the BS3 stages run
+    the fused stage kernels (u + a*k and the candidate formed while loading)."""
     import sgnx
     from sgnx import cases
     case = cases.gate(Nx=13, Ny=6, bc=("periodic", "periodic", "wall", "wall"))
     s = sgnx.SGN2D(**case.solver_kwargs())
+    s.set_rhs_kernel(variant, rows)
     o = OL.Oracle.from_case(case)
     dt = 0.5 * o.cfl_time_step(case.U0, 0.5)
     s.upload(case.U0)
@@ -54,9 +58,10 @@ def test_diagnostics():
     s.close()
 
 
-def _adaptive(case, t_end, opts_kw, max_acc=0, step_cap=4096):
+def _adaptive(case, t_end, opts_kw, max_acc=0, step_cap=4096, variant=1, rows=16):
     import sgnx
     s = sgnx.SGN2D(**case.solver_kwargs())
+    s.set_rhs_kernel(variant, rows)
     o = OL.Oracle.from_case(case)
     dt0 = o.cfl_time_step(case.U0, 0.5)
     s.upload(case.U0)
@@ -70,12 +75,15 @@ def _adaptive(case, t_end, opts_kw, max_acc=0, step_cap=4096):
     return st, sto, Ug, Uo, glog, olog
 
 
-def test_adaptive_gate_matches_oracle():
+@pytest.mark.parametrize("variant,rows", [(0, 16), (1, 16), (1, 4)])
+def test_adaptive_gate_matches_oracle(variant, rows):
     """The reference's adaptive gate (test/test_cuda_sgn2d.cu:96-128), held to
-    the stricter bar: same step counts, relative L2 <= 1e-12."""
+    the stricter bar: same step counts, relative L2 <= 1e-12 (unfused step
+    kernels with variant 0; fused stage inputs and error with variant 1)."""
     from sgnx import cases
     case = cases.gate()
-    st, sto, Ug, Uo, glog, olog = _adaptive(case, 0.3, dict(atol=1e-10, rtol=1e-8))
+    st, sto, Ug, Uo, glog, olog = _adaptive(case, 0.3, dict(atol=1e-10, rtol=1e-8),
+                                            variant=variant, rows=rows)
     assert not st.aborted and not sto.aborted
     assert (st.accepted, st.rejected, st.rhsEvals) == (sto.accepted, sto.rejected, sto.rhsEvals)
     assert rel_l2(Ug, Uo) <= 1e-12
