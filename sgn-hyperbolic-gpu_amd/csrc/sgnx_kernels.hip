@@ -266,9 +266,9 @@ k_rhs(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph
 //
 // A workgroup owns a strip of TW element columns and a chunk of `rows`
 // element rows and walks up the chunk one element row r per iteration:
-//   A  the row's nodes (loads issued during the previous iteration): stage
-//      input, primitives, nodal F and G, source -> LDS / registers; the
-//      y-interface items issue the loads of the next row's bottom-face nodes
+//   A  the row's nodes (loaded during the previous iteration; the loads of
+//      row r+1 are issued here): stage input, primitives, nodal F and G,
+//      source -> LDS / registers
 //   B  interface items: the TW+1 x-interface lines of row r, and the
 //      y-interface line r+1 on top of the row (the line at its bottom was
 //      done in the previous iteration, or before the loop). A y-interface is
@@ -279,8 +279,10 @@ k_rhs(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph
 //      F* - F(left/lower state) and F* - F(right/upper state) and the depth
 //      jumps h* - h, which are the jumps the reference forms for each of the
 //      two elements (sgn2d.cpp:251-257, :268-275); y lines are double-buffered.
-//   C  nodes: the loads of the next row are issued, then the complete RHS of
-//      row r (and, in the candidate stage, its scaled embedded error).
+//      The operands an interface item takes from global memory (face depth,
+//      halo node, the bottom face of row r+1) are also loaded one row ahead.
+//   C  nodes: the complete RHS of row r (and, in the candidate stage, its
+//      scaled embedded error).
 // Every y-interface is evaluated once; the stage input is read once per chunk,
 // plus the top faces of the row below the chunk and the x-halo nodes.
 template <int TW>
@@ -443,6 +445,45 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     y_line(ey0, SL, SR, ey0 & 1);
   }
 
+  // Interface items' global operands, loaded one row ahead: the face depth
+  // and, for the strip's two halo x-lines and for the y items, the one node
+  // state that is not in this workgroup's row (the halo node of row r, the
+  // bottom face of row r+1); fNode says whether that node exists (false at a
+  // wall side, where the ghost state replaces it).
+  const bool isX = t < nX;
+  const int xi = t / NP, xb = t - NP * xi;      // x item: line xi, node xb
+  auto face_node = [&](int r, unsigned& n) -> bool {
+    if (isX) {
+      const int ex = ex0 + xi;
+      if (xi == 0) {
+        if (ex > 0) { n = node_of(Nx, ex - 1, r, NP - 1, xb); return true; }
+        if (g.bc[0] == 0) { n = node_of(Nx, Nx - 1, r, NP - 1, xb); return true; }
+        return false;
+      }
+      if (xi == nW) {
+        if (ex < Nx) { n = node_of(Nx, ex, r, 0, xb); return true; }
+        if (g.bc[1] == 0) { n = node_of(Nx, 0, r, 0, xb); return true; }
+        return false;
+      }
+      return false;
+    }
+    const int y = r + 1;
+    if (y < Ny) { n = node_of(Nx, ex0 + jj, y, aa, 0); return true; }
+    if (g.bc[3] == 0) { n = node_of(Nx, ex0 + jj, 0, aa, 0); return true; }
+    return false;
+  };
+  StageRaw<MODE> fR;
+  double fH = 0.0;
+  auto face_prefetch = [&](int r) {
+    if (isX || isY) {
+      unsigned n;
+      if (face_node(r, n)) stage_load<MODE>(P, n, fR);
+      fH = isX ? g.HfX[((long long)r * (Nx + 1) + ex0 + xi) * NP + xb]
+               : g.HfY[((long long)(r + 1) * Nx + ex0 + jj) * NP + aa];
+    }
+  };
+  face_prefetch(ey0);
+
   // loads of row ey0's node data
   StageRaw<MODE> pR;
   double pH = 0.0, pHx = 0.0, pHy = 0.0;
@@ -462,6 +503,11 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
       double Un[NV];
       stage_combine<MODE>(pR, sa.a, Un);
       const double Hn = pH, Hxn = pHx, Hyn = pHy;
+      if (r + 1 < ey1) {                      // row r+1, consumed one iteration later
+        const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
+        stage_load<MODE>(P, gn, pR);
+        pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
+      }
       if (MODE == kBs3) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) (sa.uNew + (long long)v * Nd)[rowBase + t] = Un[v];
@@ -483,75 +529,99 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     __syncthreads();
 
     // ---- B ---------------------------------------------------------------------
-    if (t < nX) {
-      const int i = t / NP, bb = t - NP * i;
-      const int ex = ex0 + i;                   // left edge of column ex
+    // Gather the two states of this work-item's interface node (branches only
+    // load), then one code path for every item: a y-interface enters with hu
+    // and hv exchanged and its jumps leave exchanged back.
+    if (isX || isY) {
       double SL[NV], SR[NV];
-      bool ghostL = false, ghostR = false;
-      if (i < nW) {
+      double SG[NV];                            // the prefetched global node, if any
+      unsigned gnode;
+      const bool haveG = face_node(r, gnode);
+      if (haveG) {
+        stage_combine<MODE>(fR, sa.a, SG);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) SR[v] = sU[v][i * NPP + NP * bb] + 0.0;
-      } else if (ex < Nx) {
-        stage_face<MODE>(P, sa.a, node_of(Nx, ex, r, 0, bb), SR);
-      } else if (g.bc[1] == 0) {
-        stage_face<MODE>(P, sa.a, node_of(Nx, 0, r, 0, bb), SR);
-      } else {
-        ghostR = true;
+        for (int v = 0; v < NV; ++v) SG[v] = SG[v] + 0.0;
       }
-      if (i > 0) {
+      const double Hf = fH;
+      if (r + 1 < ey1) face_prefetch(r + 1);
+      int slot;
+      if (isX) {
+        const int i = xi, bb = xb;
+        bool ghostL = false, ghostR = false;
+        if (i < nW) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) SL[v] = sU[v][(i - 1) * NPP + (NP - 1) + NP * bb] + 0.0;
-      } else if (ex > 0) {
-        stage_face<MODE>(P, sa.a, node_of(Nx, ex - 1, r, NP - 1, bb), SL);
-      } else if (g.bc[0] == 0) {
-        stage_face<MODE>(P, sa.a, node_of(Nx, Nx - 1, r, NP - 1, bb), SL);
-      } else {
-        ghostL = true;
-      }
-      if (ghostL) {
-        if (g.bc[0] == 1) {
-          wall_ghost(SR, IHU, SL);
+          for (int v = 0; v < NV; ++v) SR[v] = sU[v][i * NPP + NP * bb] + 0.0;
+        } else if (haveG) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) SR[v] = SG[v];
         } else {
-          const double* c = g.inflow + ((long long)r * NP + bb) * (NV * 3);
-#pragma unroll
-          for (int v = 0; v < NV; ++v)
-            SL[v] = (c[3 * v] + c[3 * v + 1] * gt.s) + c[3 * v + 2] * gt.c;
+          ghostR = true;
         }
+        if (i > 0) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) SL[v] = sU[v][(i - 1) * NPP + (NP - 1) + NP * bb] + 0.0;
+        } else if (haveG) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) SL[v] = SG[v];
+        } else {
+          ghostL = true;
+        }
+        if (ghostL) {
+          if (g.bc[0] == 1) {
+            wall_ghost(SR, IHU, SL);
+          } else {
+            const double* c = g.inflow + ((long long)r * NP + bb) * (NV * 3);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+              SL[v] = (c[3 * v] + c[3 * v + 1] * gt.s) + c[3 * v + 2] * gt.c;
+          }
+        }
+        if (ghostR) wall_ghost(SL, IHU, SR);
+        slot = t;
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          SL[v] = sU[v][jj * NPP + aa + NP * (NP - 1)] + 0.0;
+          SR[v] = SG[v];
+        }
+        if (!haveG) wall_ghost(SL, IHV, SR);    // the top wall
+        slot = NXI + ky;
       }
-      if (ghostR) wall_ghost(SL, IHU, SR);
-      const double Hf = g.HfX[((long long)r * (Nx + 1) + ex) * NP + bb];
+      const bool xch = !isX;
+      double x0;
+#define SGNX_XCH(A) x0 = xch ? A[IHV] : A[IHU]; A[IHV] = xch ? A[IHU] : A[IHV]; A[IHU] = x0
+      SGNX_XCH(SL);
+      SGNX_XCH(SR);
       double Fs[NV], FL[NV], FR[NV];
       interface_flux<0>(SL, SR, Hf, ph, Fs, FL, FR);
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        xLo[v][t] = Fs[v] - FL[v];
-        xHi[v][t] = Fs[v] - FR[v];
-      }
+      SGNX_XCH(Fs);
+      SGNX_XCH(FL);
+      SGNX_XCH(FR);
+#undef SGNX_XCH
       const double hs = 0.5 * (SL[IH] + SR[IH]);
-      xhLo[t] = hs - SL[IH];
-      xhHi[t] = hs - SR[IH];
-    } else if (isY) {
-      double SL[NV], SU[NV];
+      if (isX) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) SL[v] = sU[v][jj * NPP + aa + NP * (NP - 1)] + 0.0;
-      y_state_above(r + 1, SU);                     // bottom face of row r+1
-      y_line(r + 1, SL, SU, (r + 1) & 1);
+        for (int v = 0; v < NV; ++v) {
+          xLo[v][slot] = Fs[v] - FL[v];
+          xHi[v][slot] = Fs[v] - FR[v];
+        }
+        xhLo[slot] = hs - SL[IH];
+        xhHi[slot] = hs - SR[IH];
+      } else {
+        const int buf = (r + 1) & 1;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          yLo[buf][v][ky] = Fs[v] - FL[v];
+          yHi[buf][v][ky] = Fs[v] - FR[v];
+        }
+        yhLo[buf][ky] = hs - SL[IH];
+        yhHi[buf][ky] = hs - SR[IH];
+      }
     }
     __syncthreads();
 
     // ---- C: dU of row r ---------------------------------------------------------
-    // loads of the next row, consumed in A: issued before the lift when the
-    // stage has one or two operands, after it for the four-operand candidate
-    // (whose 20 values would otherwise stay live across the whole phase)
-    auto prefetch_next = [&]() {
-      if (r + 1 < ey1) {
-        const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
-        stage_load<MODE>(P, gn, pR);
-        pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
-      }
-    };
     if (isNode) {
-      if (MODE != kBs3) prefetch_next();
       double Da[NP], Db[NP];
 #pragma unroll
       for (int k = 0; k < NP; ++k) { Da[k] = sD[a * NP + k]; Db[k] = sD[b * NP + k]; }
@@ -607,7 +677,6 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
           eM = (eM < ar) ? ar : eM;
         }
       }
-      if (MODE == kBs3) prefetch_next();
     }
     __syncthreads();
   }
