@@ -44,6 +44,13 @@ sys.path.insert(0, os.path.join(ROOT, "sgn-hyperbolic-gpu_amd"))
 METRIC = "DOF-RHS-evals/s at P=4 1024x512, % HBM roofline; batched and adjoint vs plain"
 HBM_PEAK_GBS = 8000.0
 ALG_BYTES_PER_NODE = 104          # 13 doubles: U (5) + dU (5) + H, Hx, Hy
+# Algorithmic bytes per node of each RHS launch kind of an adaptive step (the
+# RHS's 104 B plus the stage operands read and the results written in the same
+# launch): R(U); R(u + a k1) [u, k1 in]; R(u + a k2) with the candidate
+# [u, k2, k1 in; uNew, z out; k3 not stored]; k4 = R(uNew) with the error
+# [uNew, u, z in; k4 out].
+ALG_BYTES_BY_KIND = (104, 144, 224, 184)
+KIND_NAMES = ("plain", "stage2", "stage3_candidate", "k4_error")
 
 
 def log(*a):
@@ -176,15 +183,11 @@ def main():
     nd = case.nd
     dof = 5.0 * nd
     value = dof * st.rhsEvals / wall
-    # algorithmic bytes per node of an RHS launch by stage input: the RHS
-    # itself (104 B) plus the stage operands read and results written in the
-    # same launch (u + a*k: one more state read; candidate: three more state
-    # reads and the uNew write)
-    alg = [ALG_BYTES_PER_NODE, ALG_BYTES_PER_NODE + 40, ALG_BYTES_PER_NODE + 160]
     launches = sum(n for n, _ in timing)
     rhs_ms = sum(ms for _, ms in timing)
     mean_rhs_s = rhs_ms / launches / 1e3
-    achieved = sum(alg[m] * nd * timing[m][0] for m in range(3)) / (rhs_ms / 1e3) / 1e9
+    achieved = sum(ALG_BYTES_BY_KIND[m] * nd * timing[m][0]
+                   for m in range(len(timing))) / (rhs_ms / 1e3) / 1e9
     achieved_104 = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "rhs_traffic.json")
@@ -234,11 +237,10 @@ def main():
             "kernel": "k_rhs_sweep (FR/SGN right-hand side, stage input fused)",
             "mean_launch_ms": mean_rhs_s * 1e3,
             "launches": launches,
-            "launches_by_stage": {"plain": timing[0][0], "u+a*k": timing[1][0],
-                                  "candidate": timing[2][0]},
-            "mean_ms_by_stage": {k: (timing[m][1] / timing[m][0] if timing[m][0] else None)
-                                 for m, k in enumerate(("plain", "u+a*k", "candidate"))},
-            "alg_bytes_per_node_by_stage": {"plain": alg[0], "u+a*k": alg[1], "candidate": alg[2]},
+            "launches_by_kind": {k: timing[m][0] for m, k in enumerate(KIND_NAMES)},
+            "mean_ms_by_kind": {k: (timing[m][1] / timing[m][0] if timing[m][0] else None)
+                                for m, k in enumerate(KIND_NAMES)},
+            "alg_bytes_per_node_by_kind": dict(zip(KIND_NAMES, ALG_BYTES_BY_KIND)),
             "achieved_rhs_only_104B": achieved_104,
         },
         "cpu_baseline": cpu,

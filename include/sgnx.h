@@ -150,10 +150,13 @@ int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows);
 
 /* Launch timing (instrumentation, no counterpart in the reference): while
  * enabled, every RHS launch of this handle is bracketed by HIP events on the
- * handle's stream. sgnx_rhs_timing waits for the stream, fills launches[3] and
- * total_ms[3] (summed device milliseconds) by stage input: [0] a plain state,
- * [1] u + a*k (stages 2 and 3 of a step), [2] the BS3 candidate (stage 4,
- * which also writes uNew and the error partials), and clears the record. */
+ * handle's stream. sgnx_rhs_timing waits for the stream, fills
+ * launches[SGNX_TIMING_MODES] and total_ms[SGNX_TIMING_MODES] (summed device
+ * milliseconds) by launch kind: [0] R of a plain state, [1] R(u + a k) (stage 2
+ * of a step), [2] R(u + a k) writing the candidate uNew and the partial
+ * embedded sum instead of R (stage 3), [3] k4 = R(uNew) with the scaled error
+ * partials, and clears the record. */
+#define SGNX_TIMING_MODES 4
 int sgnx_enable_timing(sgnx_solver* s, int on);
 int sgnx_rhs_timing(sgnx_solver* s, long* launches, double* total_ms);
 

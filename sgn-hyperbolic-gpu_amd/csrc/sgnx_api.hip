@@ -184,7 +184,7 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   if (const char* v = std::getenv("SGNX_RHS_VARIANT")) g.variant = std::atoi(v) == 0 ? 0 : 1;
   if (const char* r = std::getenv("SGNX_SWEEP_ROWS")) g.rows = std::max(1, std::atoi(r));
   g.waves3 = 0x1;   // plain at 3 waves/SIMD; the stages with more operands at 2
-  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) g.waves3 = std::atoi(w) & 0x7;
+  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) g.waves3 = std::atoi(w) & 0xf;
   for (int k = 0; k < 4; ++k) g.bc[k] = d->bc[k];
   Phys& ph = s->ph;
   ph.g = d->par.g; ph.c = d->par.c; ph.gamma = d->par.gamma; ph.nManning = d->par.nManning;
@@ -326,6 +326,37 @@ int sgnx_rhs_once(sgnx_solver* s, double* dU, size_t n, double t) {
   return sync(s);
 }
 
+// The BS3 stages with the row-sweep kernel (rk.cpp:99-131): k2, then the k3
+// launch writing the candidate d_uNew and the partial embedded sum (into d_k3),
+// then, when `withErr`, k4 = R(uNew) into d_k4 with the scaled error partials
+// in d_part.
+static int fused_bs3(sgnx_solver* s, double t, double dt, double atol, double rtol,
+                     bool withErr) {
+  int rc;
+  Stage sa{};
+  sa.u = s->d_U;
+  sa.k1 = s->d_k1;
+  sa.a = 0.5 * dt;
+  if ((rc = rhs_launch(s, kAxpy, sa, s->d_k2, t + 0.5 * dt))) return rc;
+  sa.k1 = s->d_k2;
+  sa.a = 0.75 * dt;
+  sa.kc = s->d_k1;
+  sa.dt = dt;
+  sa.uNew = s->d_uNew;
+  sa.zOut = s->d_k3;
+  if ((rc = rhs_launch(s, kCand, sa, nullptr, t + 0.75 * dt))) return rc;
+  if (!withErr) return SGNX_OK;
+  Stage se{};
+  se.u = s->d_uNew;
+  se.u0 = s->d_U;
+  se.z = s->d_k3;
+  se.dt = dt;
+  se.atol = atol;
+  se.rtol = rtol;
+  se.part = s->d_part;
+  return rhs_launch(s, kErr, se, s->d_k4, t + dt);
+}
+
 int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgnx_rk_opts* o,
                    sgnx_rk_stats* st, double* step_log, long cap) {
   if (!s || !o || !st) return SGNX_EINVAL;
@@ -348,24 +379,8 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
     if (o->max_accepted > 0 && S.accepted >= o->max_accepted) break;
     if (t + dt > tEnd) dt = tEnd - t;
     if (s->g.variant == 1) {
-      // stage inputs, the candidate and its embedded error fused into the RHS
-      Stage sa{};
-      sa.u = s->d_U;
-      sa.k1 = s->d_k1;
-      sa.a = 0.5 * dt;
-      if ((rc = rhs_launch(s, kAxpy, sa, s->d_k2, t + 0.5 * dt))) return rc;
-      sa.k1 = s->d_k2;
-      sa.a = 0.75 * dt;
-      if ((rc = rhs_launch(s, kAxpy, sa, s->d_k3, t + 0.75 * dt))) return rc;
-      sa.k1 = s->d_k1;
-      sa.k2 = s->d_k2;
-      sa.k3 = s->d_k3;
-      sa.a = dt;
-      sa.uNew = s->d_uNew;
-      sa.atol = o->atol;
-      sa.rtol = o->rtol;
-      sa.part = s->d_part;
-      if ((rc = rhs_launch(s, kBs3, sa, s->d_k4, t + dt))) return rc;
+      // stage inputs, the candidate and the embedded error fused into the RHS
+      if ((rc = fused_bs3(s, t, dt, o->atol, o->rtol, true))) return rc;
       HIPCHK(s, launch_finish(s->d_part, sweep_tiles(s->g), s->d_out, s->stream));
     } else {
       HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
@@ -448,25 +463,9 @@ int sgnx_bs3_step(sgnx_solver* s, double t, double dt) {
   int rc;
   if ((rc = rhs_dev(s, s->d_U, s->d_k1, t))) return rc;
   if (s->g.variant == 1) {
-    // the fused stage kernels of the adaptive loop (the candidate launch also
-    // evaluates k4 and the error, which this fixed step does not use)
-    Stage sa{};
-    sa.u = s->d_U;
-    sa.k1 = s->d_k1;
-    sa.a = 0.5 * dt;
-    if ((rc = rhs_launch(s, kAxpy, sa, s->d_k2, t + 0.5 * dt))) return rc;
-    sa.k1 = s->d_k2;
-    sa.a = 0.75 * dt;
-    if ((rc = rhs_launch(s, kAxpy, sa, s->d_k3, t + 0.75 * dt))) return rc;
-    sa.k1 = s->d_k1;
-    sa.k2 = s->d_k2;
-    sa.k3 = s->d_k3;
-    sa.a = dt;
-    sa.uNew = s->d_uNew;
-    sa.atol = 1.0;
-    sa.rtol = 0.0;
-    sa.part = s->d_part;
-    if ((rc = rhs_launch(s, kBs3, sa, s->d_k4, t + dt))) return rc;
+    // the fused stage kernels of the adaptive loop, without the last RHS
+    // (a fixed step has no use for k4 or the error)
+    if ((rc = fused_bs3(s, t, dt, 1.0, 0.0, false))) return rc;
   } else {
     HIPCHK(s, launch_comb1(s->d_tmp, s->d_U, 0.5 * dt, s->d_k1, n, s->stream));
     if ((rc = rhs_dev(s, s->d_tmp, s->d_k2, t + 0.5 * dt))) return rc;
@@ -536,7 +535,7 @@ int sgnx_rhs_timing(sgnx_solver* s, long* launches, double* total_ms) {
   if (!s || !launches || !total_ms) return SGNX_EINVAL;
   int rc = sync(s);
   if (rc) return rc;
-  for (int m = 0; m < 3; ++m) { launches[m] = 0; total_ms[m] = 0.0; }
+  for (int m = 0; m < SGNX_TIMING_MODES; ++m) { launches[m] = 0; total_ms[m] = 0.0; }
   for (long i = 0; i < s->timed; ++i) {
     float ms = 0.0f;
     HIPCHK(s, hipEventElapsedTime(&ms, s->events[2 * i], s->events[2 * i + 1]));

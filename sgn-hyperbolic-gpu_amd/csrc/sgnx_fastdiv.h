@@ -50,13 +50,34 @@ inline ConstDiv make_const_div(double b) {
   return d;
 }
 
-SGNX_HD inline double div_const(double a, const ConstDiv& d) {
+// a is in the range the fast sequence is exact for
+SGNX_HD inline bool div_fast_ok(double a) {
   const double aa = fabs(a);
-  if (!d.fast || !(aa <= 0x1p+800) || (aa < 0x1p-800 && aa != 0.0)) return a / d.b;
+  return aa <= 0x1p+800 && (aa >= 0x1p-800 || aa == 0.0);
+}
+
+SGNX_HD inline double div_fast(double a, const ConstDiv& d) {
   const double t = a * d.rl;
   const double q1 = fma(a, d.rh, t);
   const double r = fma(-q1, d.b, a);
   return -fma(-r, d.rh, -q1);
+}
+
+SGNX_HD inline double div_const(double a, const ConstDiv& d) {
+  if (!d.fast || !div_fast_ok(a)) return a / d.b;
+  return div_fast(a, d);
+}
+
+// Two quotients behind one range test (one branch instead of two on a GPU).
+SGNX_HD inline void div_const2(double a0, double a1, const ConstDiv& d0, const ConstDiv& d1,
+                               double& q0, double& q1) {
+  if ((d0.fast & d1.fast) && div_fast_ok(a0) && div_fast_ok(a1)) {
+    q0 = div_fast(a0, d0);
+    q1 = div_fast(a1, d1);
+  } else {
+    q0 = a0 / d0.b;
+    q1 = a1 / d1.b;
+  }
 }
 
 }  // namespace sgnx

@@ -49,23 +49,33 @@ struct Geometry {
 struct GhostTime { double s, c; };
 
 // Stage input formed while loading (row-sweep kernel), with the arithmetic of
-// src/solver/rk.cpp:100-108:
-//   kPlain   U
-//   kAxpy    u + a*k1                                (stages 2 and 3)
-//   kBs3     u + a*((2/9 k1 + 1/3 k2) + 4/9 k3)      (candidate uNew, a = dt);
-//            the launch also writes uNew and the scaled embedded error of
-//            rk.cpp:118-131 with its own output as k4, one (sum, carry, max)
-//            triple per workgroup in `part`
-enum StageMode { kPlain = 0, kAxpy = 1, kBs3 = 2 };
+// src/solver/rk.cpp:99-131 (BS3 stages, candidate and embedded error):
+//   kPlain   U                                   dU = R(U)
+//   kAxpy    u + a*k1                            dU = R(u + a k1)   (stage 2)
+//   kCand    u + a*k1, a = 0.75 dt, k1 = k2      stage 3; at each node the
+//            launch forms k3 = R(.) and writes, instead of k3,
+//              uNew = u + dt*((2/9 kc + 1/3 k1) + 4/9 k3)   (kc = the k1 of rk.cpp)
+//              z    = (7/24 kc + 0.25 k1) + 1/3 k3
+//            (the candidate and the first three terms of the embedded sum)
+//   kErr     uNew                                dU = k4 = R(uNew), and the
+//            scaled embedded error with u0 = u and z:
+//              r = (uNew - (u0 + dt*(z + 0.125 k4))) / (atol + rtol*max|.|),
+//            one (sum r^2, carry, max |r|) triple per workgroup in `part`
+// Together kCand and kErr read and write what the separate candidate update
+// would, but every face and halo state of the last RHS is a single operand.
+enum StageMode { kPlain = 0, kAxpy = 1, kCand = 2, kErr = 3 };
 struct Stage {
-  const double* u;
-  const double* k1;
-  const double* k2;
-  const double* k3;
-  double a;
-  double* uNew;
-  double atol, rtol;
-  double* part;
+  const double* u;    // stage input (kErr: uNew)
+  const double* k1;   // kAxpy, kCand: the increment operand
+  double a;           // kAxpy, kCand: its coefficient
+  const double* kc;   // kCand: k1 of the step
+  double dt;          // kCand, kErr
+  double* uNew;       // kCand: candidate out
+  double* zOut;       // kCand: partial embedded sum out
+  const double* u0;   // kErr: u of the step
+  const double* z;    // kErr: partial embedded sum
+  double atol, rtol;  // kErr
+  double* part;       // kErr
 };
 
 // ---- launchers (sgnx_kernels.hip); all asynchronous on `st` ----------------
@@ -73,8 +83,9 @@ struct Stage {
 // dU = R(U)
 hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double* dU,
                       GhostTime gt, hipStream_t st);
-// dU = R(stage input) with the row-sweep kernel; mode kBs3 also writes
-// sa.uNew and sa.part (3 doubles per workgroup, folded by launch_finish).
+// dU = R(stage input) with the row-sweep kernel (kCand writes sa.uNew and
+// sa.zOut instead of dU; kErr also writes sa.part, 3 doubles per workgroup,
+// folded by launch_finish).
 hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const Stage& sa,
                             double* dU, GhostTime gt, hipStream_t st);
 // out[0] = sum, out[1] = max of nb (sum, carry, max) partial triples, fixed order

@@ -292,10 +292,18 @@ struct Sweep {
   static constexpr int NYI = NP * TW;         // y-interface nodes per line
 };
 
-// Raw operands of the stage input at one node (5 variables each).
+// Operands of the stage input at one node (5 variables each): u, and the
+// increment k of u + a*k in the modes that have one.
 template <int MODE>
 struct StageRaw {
-  double u[NV], k1[MODE >= 1 ? NV : 1], k2[MODE == 2 ? NV : 1], k3[MODE == 2 ? NV : 1];
+  static constexpr bool kHasK = MODE == kAxpy || MODE == kCand;
+  double u[NV], k[kHasK ? NV : 1];
+};
+// Per-node operands beyond the stage input: kCand: x0 = kc; kErr: x0 = u0,
+// x1 = z. Faces and halos never need them.
+template <int MODE>
+struct NodeExtra {
+  double x0[(MODE == kCand || MODE == kErr) ? NV : 1], x1[MODE == kErr ? NV : 1];
 };
 
 // Uniform per-variable base pointers: a node's value is base[v][node] with a
@@ -312,7 +320,7 @@ __device__ __forceinline__ VarPtrs var_ptrs(const double* base, long long Nd) {
 }
 
 struct StagePtrs {
-  VarPtrs u, k1, k2, k3;
+  VarPtrs u, k, x0, x1;
 };
 
 template <int MODE>
@@ -320,22 +328,26 @@ __device__ __forceinline__ void stage_load(const StagePtrs& P, unsigned node, St
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     R.u[v] = P.u.p[v][node];
-    if (MODE >= kAxpy) R.k1[v] = P.k1.p[v][node];
-    if (MODE == kBs3) {
-      R.k2[v] = P.k2.p[v][node];
-      R.k3[v] = P.k3.p[v][node];
-    }
+    if (StageRaw<MODE>::kHasK) R.k[v] = P.k.p[v][node];
   }
 }
 
-// rk.cpp:100-108: the stage input from its operands
+template <int MODE>
+__device__ __forceinline__ void extra_load(const StagePtrs& P, unsigned node, NodeExtra<MODE>& E) {
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (MODE == kCand || MODE == kErr) E.x0[v] = P.x0.p[v][node];
+    if (MODE == kErr) E.x1[v] = P.x1.p[v][node];
+  }
+}
+
+// rk.cpp:100, :102: the stage input u + (c dt) k from its operands (a = c dt)
 template <int MODE>
 __device__ __forceinline__ void stage_combine(const StageRaw<MODE>& R, double a, double* W) {
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    if (MODE == kPlain) W[v] = R.u[v];
-    else if (MODE == kAxpy) W[v] = R.u[v] + a * R.k1[v];
-    else W[v] = R.u[v] + a * (2.0 / 9.0 * R.k1[v] + 1.0 / 3.0 * R.k2[v] + 4.0 / 9.0 * R.k3[v]);
+    if (StageRaw<MODE>::kHasK) W[v] = R.u[v] + a * R.k[v];
+    else W[v] = R.u[v];
   }
 }
 
@@ -347,6 +359,12 @@ __device__ __forceinline__ void stage_face(const StagePtrs& P, double a, unsigne
   stage_combine<MODE>(R, a, W);
 #pragma unroll
   for (int v = 0; v < NV; ++v) W[v] = W[v] + 0.0;
+}
+
+// S[v] for a v that may not be a compile-time constant (selects, no
+// indexed register access)
+__device__ __forceinline__ double pick(const double* S, int v) {
+  return (v == IH) ? S[IH] : (v == IHU) ? S[IHU] : (v == IHV) ? S[IHV] : (v == IHW) ? S[IHW] : S[IHP];
 }
 
 __device__ __forceinline__ unsigned node_of(int Nx, int ex, int ey, int a, int b) {
@@ -386,9 +404,9 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   const int Nx = g.Nx, Ny = g.Ny;
   StagePtrs P;
   P.u = var_ptrs(MODE == kPlain ? U : sa.u, Nd);
-  P.k1 = var_ptrs(sa.k1, Nd);
-  P.k2 = var_ptrs(sa.k2, Nd);
-  P.k3 = var_ptrs(sa.k3, Nd);
+  P.k = var_ptrs(sa.k1, Nd);
+  P.x0 = var_ptrs(MODE == kCand ? sa.kc : sa.u0, Nd);
+  P.x1 = var_ptrs(sa.z, Nd);
 
   // node work item: node (a, b) of element j of the strip
   const int j = t / NPP, loc = t - NPP * j;
@@ -486,31 +504,40 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
 
   // loads of row ey0's node data
   StageRaw<MODE> pR;
+  NodeExtra<MODE> pE;
   double pH = 0.0, pHx = 0.0, pHy = 0.0;
   if (isNode) {
     const unsigned gn = node_of(Nx, ex0, ey0, 0, 0) + t;
     stage_load<MODE>(P, gn, pR);
+    extra_load<MODE>(P, gn, pE);
     pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
   }
-  // scaled embedded error of this work-item's nodes (candidate stage)
+  // scaled embedded error of this work-item's nodes (kErr)
   double eS = 0.0, eC = 0.0, eM = 0.0;
 
   for (int r = ey0; r < ey1; ++r) {
     const unsigned rowBase = node_of(Nx, ex0, r, 0, 0);
     // ---- A ---------------------------------------------------------------------
     double S[NV], qu = 0.0, qv = 0.0;
+    // per-node operands phase C needs beyond the RHS (rk.cpp:106-108, :119-120,
+    // sums evaluated left to right as there):
+    //   kCand: c0 = u, c1 = 2/9 kc + 1/3 k, c2 = 7/24 kc + 0.25 k
+    //   kErr:  c0 = u0, c2 = z
+    double c0[NV], c1[NV], c2[NV];
     if (isNode) {
       double Un[NV];
       stage_combine<MODE>(pR, sa.a, Un);
       const double Hn = pH, Hxn = pHx, Hyn = pHy;
-      if (r + 1 < ey1) {                      // row r+1, consumed one iteration later
-        const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
-        stage_load<MODE>(P, gn, pR);
-        pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
-      }
-      if (MODE == kBs3) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) (sa.uNew + (long long)v * Nd)[rowBase + t] = Un[v];
+      for (int v = 0; v < NV; ++v) {
+        if (MODE == kCand) {
+          c0[v] = pR.u[v];
+          c1[v] = 2.0 / 9.0 * pE.x0[v] + 1.0 / 3.0 * pR.k[v];
+          c2[v] = 7.0 / 24.0 * pE.x0[v] + 0.25 * pR.k[v];
+        } else if (MODE == kErr) {
+          c0[v] = pE.x0[v];
+          c2[v] = pE.x1[v];
+        }
       }
       const Prim q = prims(Un);
       qu = q.u;
@@ -525,6 +552,17 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
         sF[v][t] = F[v];
         sG[v][t] = G[v];
       }
+    }
+    // row r+1's node operands, consumed one iteration later. Issued after
+    // the last use of this row's, and fenced against the scheduler hoisting
+    // them above those uses: the loads then land in the registers the loop
+    // carries, so no copy (and no wait for them) is needed at the back-edge.
+    __builtin_amdgcn_sched_barrier(0);
+    if (isNode && r + 1 < ey1) {
+      const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
+      stage_load<MODE>(P, gn, pR);
+      extra_load<MODE>(P, gn, pE);
+      pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
     }
     __syncthreads();
 
@@ -543,7 +581,6 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
         for (int v = 0; v < NV; ++v) SG[v] = SG[v] + 0.0;
       }
       const double Hf = fH;
-      if (r + 1 < ey1) face_prefetch(r + 1);
       int slot;
       if (isX) {
         const int i = xi, bb = xb;
@@ -617,6 +654,9 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
         yhLo[buf][ky] = hs - SL[IH];
         yhHi[buf][ky] = hs - SR[IH];
       }
+      // the next row's interface operands, after the last use of this row's
+      __builtin_amdgcn_sched_barrier(0);
+      if (r + 1 < ey1) face_prefetch(r + 1);
     }
     __syncthreads();
 
@@ -643,9 +683,7 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
       const double Bx = ph.mc2 * qu * hx;
       const double By = ph.mc2 * qv * hy;
       const unsigned gn = rowBase + t;
-      // one variable per trip keeps the LDS operands of a single variable live
-#pragma unroll 1
-      for (int v = 0; v < NV; ++v) {
+      auto variable = [&](int v) {
         double dfx = 0.0, dgy = 0.0;
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
@@ -656,19 +694,21 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
         const double cy = yHi[bot][v][iy] * gLb + yLo[top][v][iy] * gRb;
         double o = -div_const(dfx + cx, g.dx) - div_const(dgy + cy, g.dy);
         if (v == IHP) o = o - Bx - By;
-        const double sv = (v == IH) ? S[IH] : (v == IHU) ? S[IHU] : (v == IHV) ? S[IHV]
-                        : (v == IHW) ? S[IHW] : S[IHP];
-        const double k4 = o + sv;
+        const double k4 = o + pick(S, v);
         // per-variable pointers formed here (a loop-indexed pointer array
         // would live in scratch memory)
         const long long vo = (long long)v * Nd;
-        (dU + vo)[gn] = k4;
-        if (MODE == kBs3) {
-          // rk.cpp:118-131 for this entry
-          const double u0 = (sa.u + vo)[gn], un = sU[v][t];
-          const double uEmb = u0 + sa.a * (7.0 / 24.0 * (sa.k1 + vo)[gn] +
-                                           0.25 * (sa.k2 + vo)[gn] +
-                                           1.0 / 3.0 * (sa.k3 + vo)[gn] + 0.125 * k4);
+        if (MODE == kCand) {
+          // rk.cpp:106-108 and the first three terms of :119-120 (k4 is k3 here)
+          (sa.uNew + vo)[gn] = pick(c0, v) + sa.dt * (pick(c1, v) + 4.0 / 9.0 * k4);
+          (sa.zOut + vo)[gn] = pick(c2, v) + 1.0 / 3.0 * k4;
+        } else {
+          (dU + vo)[gn] = k4;
+        }
+        if (MODE == kErr) {
+          // rk.cpp:119-129 for this entry
+          const double u0 = pick(c0, v), un = sU[v][t];
+          const double uEmb = u0 + sa.dt * (pick(c2, v) + 0.125 * k4);
           const double e = un - uEmb;
           const double au = fabs(u0), an = fabs(un);
           const double r_ = e / (sa.atol + sa.rtol * ((au < an) ? an : au));
@@ -676,12 +716,27 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
           const double ar = fabs(r_);
           eM = (eM < ar) ? ar : eM;
         }
+      };
+      if (MODE == kPlain || MODE == kAxpy) {
+        // one variable per trip keeps the LDS operands of a single variable
+        // live (the register budget of three waves per SIMD)
+#pragma unroll 1
+        for (int v = 0; v < NV; ++v) variable(v);
+      } else {
+        // unrolled, so that the per-node operands stay in registers without
+        // selects; the scheduling fence keeps each variable's LDS operands
+        // from being hoisted into the previous one
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          __builtin_amdgcn_sched_barrier(0);
+          variable(v);
+        }
       }
     }
     __syncthreads();
   }
 
-  if (MODE == kBs3) {
+  if (MODE == kErr) {
     // fixed-order reduction over the workgroup (LDS reused)
     double* ss = &sF[0][0];
     double* sc = &sF[1][0];
@@ -894,11 +949,13 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
   // waves per SIMD the register allocation targets, per stage mode (bit m of
   // g.waves3 set: 3 waves, else 2)
   const bool w3 = (g.waves3 >> mode) & 1;
+  if (mode < kPlain || mode > kErr) return hipErrorInvalidValue;
 #define SGNX_SWEEP(M, W) \
   hipLaunchKernelGGL((k_rhs_sweep<kTileW, M, W>), grid, block, 0, st, sa.u, dU, g, ph, gt, sa, strips)
   if (mode == kPlain) { if (w3) SGNX_SWEEP(kPlain, 3); else SGNX_SWEEP(kPlain, 2); }
   else if (mode == kAxpy) { if (w3) SGNX_SWEEP(kAxpy, 3); else SGNX_SWEEP(kAxpy, 2); }
-  else { if (w3) SGNX_SWEEP(kBs3, 3); else SGNX_SWEEP(kBs3, 2); }
+  else if (mode == kCand) { if (w3) SGNX_SWEEP(kCand, 3); else SGNX_SWEEP(kCand, 2); }
+  else { if (w3) SGNX_SWEEP(kErr, 3); else SGNX_SWEEP(kErr, 2); }
 #undef SGNX_SWEEP
   return hipGetLastError();
 }

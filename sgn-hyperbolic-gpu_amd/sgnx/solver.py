@@ -21,6 +21,7 @@ import numpy as np
 from . import _lib as L
 
 NVAR = 5
+TIMING_MODES = 4          # launch kinds of sgnx_rhs_timing (SGNX_TIMING_MODES)
 H, HU, HV, HW, HP = range(5)
 
 _BC_CODES = {"periodic": L.BC_PERIODIC, "wall": L.BC_WALL, "inflow": L.BC_INFLOW}
@@ -275,11 +276,12 @@ class SGN2D:
 
     def rhs_timing(self):
         """[(launches, total device ms)] of RHS launches since timing was enabled,
-        by stage input: plain state, u + a*k, BS3 candidate."""
-        n = (C.c_long * 3)()
-        ms = (C.c_double * 3)()
+        by launch kind: plain state, stage 2 (u + a*k), stage 3 (u + a*k writing
+        the candidate), k4 with the embedded error."""
+        n = (C.c_long * TIMING_MODES)()
+        ms = (C.c_double * TIMING_MODES)()
         L.check(self.lib.sgnx_rhs_timing(self.h, n, ms), self.h)
-        return [(n[m], ms[m]) for m in range(3)]
+        return [(n[m], ms[m]) for m in range(TIMING_MODES)]
 
     def close(self):
         if getattr(self, "h", None):
