@@ -290,6 +290,7 @@ struct Sweep {
   static constexpr int NT = ((NPP * TW + 63) / 64) * 64;
   static constexpr int NXI = NP * (TW + 1);   // x-interface nodes per row
   static constexpr int NYI = NP * TW;         // y-interface nodes per line
+  static_assert(2 * (NXI + NYI) <= NT, "two lanes per interface node");
 };
 
 // Operands of the stage input at one node (5 variables each): u, and the
@@ -367,6 +368,18 @@ __device__ __forceinline__ double pick(const double* S, int v) {
   return (v == IH) ? S[IH] : (v == IHU) ? S[IHU] : (v == IHV) ? S[IHV] : (v == IHW) ? S[IHW] : S[IHP];
 }
 
+// Lane permutes within each group of four lanes (DPP quad_perm): the value
+// of the even lane of the pair, of the odd lane, and of the other lane.
+template <int CTRL>
+__device__ __forceinline__ double lane_perm(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_even(double x) { return lane_perm<0xA0>(x); }  // [0,0,2,2]
+__device__ __forceinline__ double lane_odd(double x) { return lane_perm<0xF5>(x); }   // [1,1,3,3]
+__device__ __forceinline__ double lane_swap(double x) { return lane_perm<0xB1>(x); }  // [1,0,3,2]
+
 __device__ __forceinline__ unsigned node_of(int Nx, int ex, int ey, int a, int b) {
   return ((unsigned)ey * (unsigned)Nx + (unsigned)ex) * NPP + a + NP * b;
 }
@@ -379,10 +392,10 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   __shared__ double sU[NV][NT];
   __shared__ double sF[NV][NT];
   __shared__ double sG[NV][NT];
-  __shared__ double xLo[NV][NXI], xHi[NV][NXI];        // F* - F(left),  F* - F(right)
-  __shared__ double xhLo[NXI], xhHi[NXI];              // h* - h(left),  h* - h(right)
-  __shared__ double yLo[2][NV][NYI], yHi[2][NV][NYI];  // F* - F(lower), F* - F(upper)
-  __shared__ double yhLo[2][NYI], yhHi[2][NYI];
+  // interface jumps by side s (0 = left / lower, 1 = right / upper):
+  // F* - F(side s) and h* - h(side s); y lines double-buffered
+  __shared__ double xJ[2][NV][NXI], xhJ[2][NXI];
+  __shared__ double yJ[2][2][NV][NYI], yhJ[2][2][NYI];
   __shared__ double sD[NPP], sgL[NP], sgR[NP];
 
   if (threadIdx.x < NPP) sD[threadIdx.x] = g.ops->D[threadIdx.x];
@@ -398,7 +411,6 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   const int ey0 = chunk * g.rows;
   const int ey1 = min(g.Ny, ey0 + g.rows);
   const int nNodes = NPP * nW;
-  const int nX = NP * (nW + 1);
   const int t = threadIdx.x;
   const long long Nd = g.Nd;
   const int Nx = g.Nx, Ny = g.Ny;
@@ -412,84 +424,107 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   const int j = t / NPP, loc = t - NPP * j;
   const int b = loc / NP, a = loc - NP * b;
   const bool isNode = t < nNodes;
-  // y-interface work item: column jj, line node aa (items NXI .. NXI + 5 nW - 1)
-  const int ky = t - NXI;
-  const bool isY = ky >= 0 && ky < NP * nW;
-  const int jj = isY ? ky / NP : 0, aa = isY ? ky - NP * jj : 0;
+  // Interface half-items: two lanes per interface node, lane 2k + side
+  // taking one side (0 = left / lower, 1 = right / upper) of interface k.
+  // Items 0 .. 2 nX - 1 are the x-interface nodes of the row (k = i NP + bb:
+  // line i on the left edge of strip column i, node bb); the next 2 NP nW items
+  // the y-interface nodes of the line above the row (k = nX + jj NP + aa).
+  // Each lane evaluates its own side (primitives, one-sided flux, wave speed)
+  // and the pair exchanges them with lane permutes, so the interface work is
+  // spread over every wave of the workgroup.
+  const int nX = NP * (nW + 1);
+  const int side = t & 1, kk = t >> 1;
+  const bool isX = t < 2 * nX;
+  const bool isY = !isX && t < 2 * nX + 2 * NP * nW;
+  const int xi = isX ? kk / NP : 0, xb = isX ? kk - NP * xi : 0;
+  const int ky = isY ? kk - nX : 0;
+  const int jj = ky / NP, aa = ky - NP * jj;
 
-  // y-interface line y between rows y-1 and y: jumps into buffer `buf`
-  auto y_line = [&](int y, const double* SLin, const double* SRin, int buf) {
-    double SL[NV], SR[NV];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) { SL[v] = SLin[v]; SR[v] = SRin[v]; }
-    if (y == 0 && g.bc[2] != 0) wall_ghost(SR, IHV, SL);
-    if (y == Ny && g.bc[3] != 0) wall_ghost(SL, IHV, SR);
-    const double Hf = g.HfY[((long long)y * Nx + ex0 + jj) * NP + aa];
-    // exchange hu <-> hv: the y-interface as an x-interface
-    double x;
-    x = SL[IHU]; SL[IHU] = SL[IHV]; SL[IHV] = x;
-    x = SR[IHU]; SR[IHU] = SR[IHV]; SR[IHV] = x;
-    double Fs[NV], FL[NV], FR[NV];
-    interface_flux<0>(SL, SR, Hf, ph, Fs, FL, FR);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const int w = (v == IHU) ? IHV : (v == IHV) ? IHU : v;
-      yLo[buf][w][ky] = Fs[v] - FL[v];
-      yHi[buf][w][ky] = Fs[v] - FR[v];
-    }
-    const double hs = 0.5 * (SL[IH] + SR[IH]);
-    yhLo[buf][ky] = hs - SL[IH];
-    yhHi[buf][ky] = hs - SR[IH];
-  };
-
-  // lower / upper face states of a y line from global memory (wrapped rows;
-  // at a wall side the state is replaced by the ghost inside y_line)
-  auto y_state_below = [&](int y, double* W) {       // top face of row y-1
-    if (y > 0) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, y - 1, aa, NP - 1), W);
-    else if (g.bc[2] == 0) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, Ny - 1, aa, NP - 1), W);
-    else for (int v = 0; v < NV; ++v) W[v] = 0.0;
-  };
-  auto y_state_above = [&](int y, double* W) {       // bottom face of row y
-    if (y < Ny) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, y, aa, 0), W);
-    else if (g.bc[3] == 0) stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, 0, aa, 0), W);
-    else for (int v = 0; v < NV; ++v) W[v] = 0.0;
-  };
-
-  // prologue: the y line at the bottom of the chunk
-  if (isY) {
-    double SL[NV], SR[NV];
-    y_state_below(ey0, SL);
-    y_state_above(ey0, SR);
-    y_line(ey0, SL, SR, ey0 & 1);
-  }
-
-  // Interface items' global operands, loaded one row ahead: the face depth
-  // and, for the strip's two halo x-lines and for the y items, the one node
-  // state that is not in this workgroup's row (the halo node of row r, the
-  // bottom face of row r+1); fNode says whether that node exists (false at a
-  // wall side, where the ghost state replaces it).
-  const bool isX = t < nX;
-  const int xi = t / NP, xb = t - NP * xi;      // x item: line xi, node xb
+  // The global node this lane's side takes for row r, if any: the halo node
+  // of the strip's outer x-lines, or the bottom face of row r+1 for the upper
+  // side of a y item (false at a wall, where a ghost replaces it, and for
+  // every side that lies in the workgroup's row).
   auto face_node = [&](int r, unsigned& n) -> bool {
     if (isX) {
-      const int ex = ex0 + xi;
-      if (xi == 0) {
-        if (ex > 0) { n = node_of(Nx, ex - 1, r, NP - 1, xb); return true; }
+      if (side == 0 && xi == 0) {
+        if (ex0 > 0) { n = node_of(Nx, ex0 - 1, r, NP - 1, xb); return true; }
         if (g.bc[0] == 0) { n = node_of(Nx, Nx - 1, r, NP - 1, xb); return true; }
         return false;
       }
-      if (xi == nW) {
+      if (side == 1 && xi == nW) {
+        const int ex = ex0 + nW;
         if (ex < Nx) { n = node_of(Nx, ex, r, 0, xb); return true; }
         if (g.bc[1] == 0) { n = node_of(Nx, 0, r, 0, xb); return true; }
         return false;
       }
       return false;
     }
+    if (side == 0) return false;
     const int y = r + 1;
     if (y < Ny) { n = node_of(Nx, ex0 + jj, y, aa, 0); return true; }
     if (g.bc[3] == 0) { n = node_of(Nx, ex0 + jj, 0, aa, 0); return true; }
     return false;
   };
+
+  // One interface node evaluated by a lane pair. W is this lane's side's
+  // state in the interface frame (a y interface has hu and hv exchanged:
+  // fluxY(U) is fluxX of the exchanged state with hu and hv of the result
+  // exchanged back, and |v| is the exchanged |u|). Returns this side's flux
+  // jump F* - F(W) (frame order) and depth jump h* - h(W), with the Rusanov
+  // flux of sgn_flux2d.h:169-179: F* = 0.5 (FL + FR) - 0.5 max(sL, sR) (UR - UL).
+  auto half_iface = [&](const double* W, double Hf, double* jf, double& jh) {
+    const Prim q = prims(W);
+    double F[NV];
+    flux_x(W, q, Hf, ph, F);
+    const double so = fabs(q.u) + signal_speed(W[IH], q, ph);
+    const double lam = fmax(lane_even(so), lane_odd(so));
+    const double hl = 0.5 * lam;
+    double hL = 0.0, hR = 0.0;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double ul = lane_even(W[v]), ur = lane_odd(W[v]);
+      // FL + FR in either lane order (addition commutes exactly)
+      const double Fs = 0.5 * (F[v] + lane_swap(F[v])) - hl * (ur - ul);
+      jf[v] = Fs - F[v];
+      if (v == IH) { hL = ul; hR = ur; }
+    }
+    const double hs = 0.5 * (hL + hR);
+    jh = hs - W[IH];
+  };
+  auto to_frame = [&](double* W) {             // exchange hu, hv on y lanes
+    const double u_ = W[IHU], v_ = W[IHV];
+    W[IHU] = isY ? v_ : u_;
+    W[IHV] = isY ? u_ : v_;
+  };
+
+  // prologue: the y line at the bottom of the chunk, both sides from global
+  // memory (the bottom wall's ghost mirrors the upper side)
+  if (isY) {
+    double W[NV];
+    const int y = ey0;
+    const unsigned nUp = node_of(Nx, ex0 + jj, y, aa, 0);
+    if (side == 1) {
+      stage_face<MODE>(P, sa.a, nUp, W);
+    } else if (y > 0) {
+      stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, y - 1, aa, NP - 1), W);
+    } else if (g.bc[2] == 0) {
+      stage_face<MODE>(P, sa.a, node_of(Nx, ex0 + jj, Ny - 1, aa, NP - 1), W);
+    } else {
+      double Wu[NV];
+      stage_face<MODE>(P, sa.a, nUp, Wu);
+      wall_ghost(Wu, IHV, W);
+    }
+    to_frame(W);
+    const double Hf = g.HfY[((long long)y * Nx + ex0 + jj) * NP + aa];
+    double jf[NV], jh;
+    half_iface(W, Hf, jf, jh);
+    to_frame(jf);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) yJ[y & 1][side][v][ky] = jf[v];
+    yhJ[y & 1][side][ky] = jh;
+  }
+
+  // the interface lanes' global operands, loaded one row ahead
   StageRaw<MODE> fR;
   double fH = 0.0;
   auto face_prefetch = [&](int r) {
@@ -567,92 +602,66 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     __syncthreads();
 
     // ---- B ---------------------------------------------------------------------
-    // Gather the two states of this work-item's interface node (branches only
-    // load), then one code path for every item: a y-interface enters with hu
-    // and hv exchanged and its jumps leave exchanged back.
     if (isX || isY) {
-      double SL[NV], SR[NV];
-      double SG[NV];                            // the prefetched global node, if any
+      double W[NV];
       unsigned gnode;
       const bool haveG = face_node(r, gnode);
       if (haveG) {
-        stage_combine<MODE>(fR, sa.a, SG);
+        stage_combine<MODE>(fR, sa.a, W);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) SG[v] = SG[v] + 0.0;
+        for (int v = 0; v < NV; ++v) W[v] = W[v] + 0.0;
       }
       const double Hf = fH;
-      int slot;
       if (isX) {
-        const int i = xi, bb = xb;
-        bool ghostL = false, ghostR = false;
-        if (i < nW) {
+        // own side: a node of this row, the halo node, or a ghost of the
+        // other side's node (a wall mirrors it; the inflow side is the table)
+        const int li = (side == 0) ? xi - 1 : xi;         // strip column of the node
+        const bool inRow = (side == 0) ? xi > 0 : xi < nW;
+        if (inRow) {
+          const int n = li * NPP + (side == 0 ? NP - 1 : 0) + NP * xb;
 #pragma unroll
-          for (int v = 0; v < NV; ++v) SR[v] = sU[v][i * NPP + NP * bb] + 0.0;
-        } else if (haveG) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) SR[v] = SG[v];
-        } else {
-          ghostR = true;
-        }
-        if (i > 0) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) SL[v] = sU[v][(i - 1) * NPP + (NP - 1) + NP * bb] + 0.0;
-        } else if (haveG) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) SL[v] = SG[v];
-        } else {
-          ghostL = true;
-        }
-        if (ghostL) {
-          if (g.bc[0] == 1) {
-            wall_ghost(SR, IHU, SL);
-          } else {
-            const double* c = g.inflow + ((long long)r * NP + bb) * (NV * 3);
+          for (int v = 0; v < NV; ++v) W[v] = sU[v][n] + 0.0;
+        } else if (!haveG) {
+          if (side == 0 && g.bc[0] == 2) {
+            const double* c = g.inflow + ((long long)r * NP + xb) * (NV * 3);
 #pragma unroll
             for (int v = 0; v < NV; ++v)
-              SL[v] = (c[3 * v] + c[3 * v + 1] * gt.s) + c[3 * v + 2] * gt.c;
+              W[v] = (c[3 * v] + c[3 * v + 1] * gt.s) + c[3 * v + 2] * gt.c;
+          } else {
+            const int n = (side == 0) ? NP * xb : (nW - 1) * NPP + (NP - 1) + NP * xb;
+            double Wo[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Wo[v] = sU[v][n] + 0.0;
+            wall_ghost(Wo, IHU, W);
           }
         }
-        if (ghostR) wall_ghost(SL, IHU, SR);
-        slot = t;
       } else {
+        const int n = jj * NPP + aa + NP * (NP - 1);    // top face of row r
+        if (side == 0 || !haveG) {
+          double Wo[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          SL[v] = sU[v][jj * NPP + aa + NP * (NP - 1)] + 0.0;
-          SR[v] = SG[v];
+          for (int v = 0; v < NV; ++v) Wo[v] = sU[v][n] + 0.0;
+          if (side == 0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) W[v] = Wo[v];
+          } else {
+            wall_ghost(Wo, IHV, W);                   // the top wall
+          }
         }
-        if (!haveG) wall_ghost(SL, IHV, SR);    // the top wall
-        slot = NXI + ky;
       }
-      const bool xch = !isX;
-      double x0;
-#define SGNX_XCH(A) x0 = xch ? A[IHV] : A[IHU]; A[IHV] = xch ? A[IHU] : A[IHV]; A[IHU] = x0
-      SGNX_XCH(SL);
-      SGNX_XCH(SR);
-      double Fs[NV], FL[NV], FR[NV];
-      interface_flux<0>(SL, SR, Hf, ph, Fs, FL, FR);
-      SGNX_XCH(Fs);
-      SGNX_XCH(FL);
-      SGNX_XCH(FR);
-#undef SGNX_XCH
-      const double hs = 0.5 * (SL[IH] + SR[IH]);
+      to_frame(W);
+      double jf[NV], jh;
+      half_iface(W, Hf, jf, jh);
+      to_frame(jf);
       if (isX) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          xLo[v][slot] = Fs[v] - FL[v];
-          xHi[v][slot] = Fs[v] - FR[v];
-        }
-        xhLo[slot] = hs - SL[IH];
-        xhHi[slot] = hs - SR[IH];
+        for (int v = 0; v < NV; ++v) xJ[side][v][xi * NP + xb] = jf[v];
+        xhJ[side][xi * NP + xb] = jh;
       } else {
         const int buf = (r + 1) & 1;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          yLo[buf][v][ky] = Fs[v] - FL[v];
-          yHi[buf][v][ky] = Fs[v] - FR[v];
-        }
-        yhLo[buf][ky] = hs - SL[IH];
-        yhHi[buf][ky] = hs - SR[IH];
+        for (int v = 0; v < NV; ++v) yJ[buf][side][v][ky] = jf[v];
+        yhJ[buf][side][ky] = jh;
       }
       // the next row's interface operands, after the last use of this row's
       __builtin_amdgcn_sched_barrier(0);
@@ -675,8 +684,8 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
         dhx = dhx + Da[k] * sU[IH][rowx + k];
         dhy = dhy + Db[k] * sU[IH][coly + NP * k];
       }
-      dhx = dhx + (xhHi[ixL] * gLa + xhLo[ixR] * gRa);
-      dhy = dhy + (yhHi[bot][iy] * gLb + yhLo[top][iy] * gRb);
+      dhx = dhx + (xhJ[1][ixL] * gLa + xhJ[0][ixR] * gRa);
+      dhy = dhy + (yhJ[bot][1][iy] * gLb + yhJ[top][0][iy] * gRb);
       const double hx = div_const(dhx, g.dx), hy = div_const(dhy, g.dy);
       // the hp entries of the non-conservative products (the others are zero,
       // and subtracting zero changes no sum)
@@ -690,8 +699,8 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
           dfx = dfx + Da[k] * sF[v][rowx + k];
           dgy = dgy + Db[k] * sG[v][coly + NP * k];
         }
-        const double cx = xHi[v][ixL] * gLa + xLo[v][ixR] * gRa;
-        const double cy = yHi[bot][v][iy] * gLb + yLo[top][v][iy] * gRb;
+        const double cx = xJ[1][v][ixL] * gLa + xJ[0][v][ixR] * gRa;
+        const double cy = yJ[bot][1][v][iy] * gLb + yJ[top][0][v][iy] * gRb;
         double o = -div_const(dfx + cx, g.dx) - div_const(dgy + cy, g.dy);
         if (v == IHP) o = o - Bx - By;
         const double k4 = o + pick(S, v);
@@ -717,7 +726,7 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
           eM = (eM < ar) ? ar : eM;
         }
       };
-      if (MODE == kPlain || MODE == kAxpy) {
+      if (WAVES == 3 && (MODE == kPlain || MODE == kAxpy)) {
         // one variable per trip keeps the LDS operands of a single variable
         // live (the register budget of three waves per SIMD)
 #pragma unroll 1
