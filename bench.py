@@ -15,12 +15,15 @@ integration of exactly --steps accepted steps, timed between two device
 synchronisations (state resident in HBM before the clock starts).
 
 Also reported:
-  roofline      the RHS kernel alone: algorithmic bytes per launch (104 B per
-                node: U read, dU written, H, Hx, Hy read) over its mean device
-                duration measured with HIP events around every RHS launch of the
-                timed run, against the 8 TB/s HBM3E peak; `traffic` is the
+  roofline      the RHS kernel (k_rhs_sweep) alone: the algorithmic bytes of
+                every RHS launch of the timed run (per node: 104 B for the RHS
+                itself -- U read, dU written, H, Hx, Hy read -- plus the stage
+                operands each launch kind reads and writes, ALG_BYTES_BY_KIND)
+                over their summed device durations, measured with HIP events
+                around every launch, against the 8 TB/s HBM3E peak; `traffic` is the
                 measured HBM bytes per launch from the rocprofv3 counter runs
-                summarised in profiles/rhs_traffic.json (null if absent)
+                summarised in profiles/rhs_traffic.json (per launch kind,
+                weighted by the timed run's launch mix; null if absent)
   cpu_baseline  the CPU oracle (the reference path restated op for op) on one
                 host core over a bounded sample of the same workload: the FSAL
                 head plus two accepted steps; the same sample also checks the
@@ -189,13 +192,18 @@ def main():
     achieved = sum(ALG_BYTES_BY_KIND[m] * nd * timing[m][0]
                    for m in range(len(timing))) / (rhs_ms / 1e3) / 1e9
     achieved_104 = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
+    # copy-calibrated, per launch kind), weighted by this run's launch mix
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "rhs_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("Nx") == case.Nx and tr.get("Ny") == case.Ny:
-            traffic = tr.get("bytes_per_launch")
+        bk = tr.get("by_kind", {})
+        if (tr.get("Nx"), tr.get("Ny")) == (case.Nx, case.Ny) and all(
+                k in bk for m, k in enumerate(KIND_NAMES) if timing[m][0]):
+            traffic = sum(bk[k]["bytes"] * timing[m][0]
+                          for m, k in enumerate(KIND_NAMES) if timing[m][0]) / launches
     log(f"[bench] timed: {st.accepted} acc / {st.rejected} rej / {st.rhsEvals} RHS in "
         f"{wall:.3f}s -> {value / 1e9:.2f} GDOF-RHS/s; RHS kernel mean {mean_rhs_s * 1e3:.4f} ms "
         f"over {launches} launches = {achieved:.0f} GB/s algorithmic")
