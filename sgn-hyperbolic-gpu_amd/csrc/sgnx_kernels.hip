@@ -726,7 +726,7 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
           eM = (eM < ar) ? ar : eM;
         }
       };
-      if (WAVES == 3 && (MODE == kPlain || MODE == kAxpy)) {
+      if (WAVES == 3 && MODE == kAxpy) {
         // one variable per trip keeps the LDS operands of a single variable
         // live (the register budget of three waves per SIMD)
 #pragma unroll 1
