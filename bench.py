@@ -15,12 +15,14 @@ integration of exactly --steps accepted steps, timed between two device
 synchronisations (state resident in HBM before the clock starts).
 
 Also reported:
-  roofline      the RHS kernel (k_rhs_sweep) alone: the algorithmic bytes of
-                every RHS launch of the timed run (per node: 104 B for the RHS
-                itself -- U read, dU written, H, Hx, Hy read -- plus the stage
-                operands each launch kind reads and writes, ALG_BYTES_BY_KIND)
-                over their summed device durations, measured with HIP events
-                around every launch, against the 8 TB/s HBM3E peak; `traffic` is the
+  roofline      the RHS kernel (k_rhs_sweep): its algorithmic bytes per launch
+                (104 B per node: U read, dU written, H, Hx, Hy read) over the
+                mean duration of the timed run's RHS launches, measured with HIP
+                events around every launch, against the 8 TB/s HBM3E peak.
+                Beside it: the same launches counting the RK stage operands each
+                fused launch kind also reads and writes (ALG_BYTES_BY_KIND), and
+                the plain RHS on its own (20 launches after the timed run);
+                `traffic` is the
                 measured HBM bytes per launch from the rocprofv3 counter runs
                 summarised in profiles/rhs_traffic.json (per launch kind,
                 weighted by the timed run's launch mix; null if absent)
@@ -96,6 +98,28 @@ def profile_rhs(args):
     log(f"[bench] profile: {args.profile_rhs} RHS launches of {case.nd} nodes; "
         f"3 device copies of {args.copy_mib} MiB")
     s.close()
+
+
+def plain_rhs_rate(s, nd, n=20):
+    """The RHS kernel on its own (no stage operands): n launches R(U) on the
+    final state of the timed run, device-resident, timed with HIP events."""
+    from sgnx import hip
+    U = hip.DeviceArray.from_host(s.download())
+    dU = hip.DeviceArray(U.n)
+    for _ in range(3):
+        s.rhs_device(U.address, dU.address)
+    s.enable_timing(True)
+    s.rhs_timing()
+    for _ in range(n):
+        s.rhs_device(U.address, dU.address)
+    cnt, ms = s.rhs_timing()[0]
+    s.enable_timing(False)
+    mean_s = ms / cnt / 1e3
+    gbs = ALG_BYTES_PER_NODE * nd / mean_s / 1e9
+    log(f"[bench] plain RHS: {mean_s * 1e3:.4f} ms = {gbs:.0f} GB/s algorithmic "
+        f"({100 * gbs / HBM_PEAK_GBS:.1f} % of peak)")
+    return {"launches": cnt, "mean_ms": mean_s * 1e3, "achieved": gbs,
+            "frac": gbs / HBM_PEAK_GBS}
 
 
 def cpu_baseline(case, gpu_solver, dt0, args):
@@ -189,9 +213,12 @@ def main():
     launches = sum(n for n, _ in timing)
     rhs_ms = sum(ms for _, ms in timing)
     mean_rhs_s = rhs_ms / launches / 1e3
-    achieved = sum(ALG_BYTES_BY_KIND[m] * nd * timing[m][0]
-                   for m in range(len(timing))) / (rhs_ms / 1e3) / 1e9
-    achieved_104 = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
+    # the RHS's algorithmic bytes (104 B/node) per launch over the mean launch
+    # duration; and, beside it, the bytes each fused launch kind also moves for
+    # its RK stage operands
+    achieved = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
+    achieved_stage = sum(ALG_BYTES_BY_KIND[m] * nd * timing[m][0]
+                         for m in range(len(timing))) / (rhs_ms / 1e3) / 1e9
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
     # copy-calibrated, per launch kind), weighted by this run's launch mix
     traffic = None
@@ -206,7 +233,9 @@ def main():
                           for m, k in enumerate(KIND_NAMES) if timing[m][0]) / launches
     log(f"[bench] timed: {st.accepted} acc / {st.rejected} rej / {st.rhsEvals} RHS in "
         f"{wall:.3f}s -> {value / 1e9:.2f} GDOF-RHS/s; RHS kernel mean {mean_rhs_s * 1e3:.4f} ms "
-        f"over {launches} launches = {achieved:.0f} GB/s algorithmic")
+        f"over {launches} launches = {achieved:.0f} GB/s algorithmic (104 B/node), "
+        f"{achieved_stage:.0f} GB/s with the stage operands")
+    plain = plain_rhs_rate(s, nd)
 
     cpu = None
     if not args.no_cpu:
@@ -243,13 +272,16 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "kernel": "k_rhs_sweep (FR/SGN right-hand side, stage input fused)",
+            "alg_bytes_per_launch": ALG_BYTES_PER_NODE * nd,
             "mean_launch_ms": mean_rhs_s * 1e3,
             "launches": launches,
+            "achieved_incl_stage_operands": achieved_stage,
+            "frac_incl_stage_operands": achieved_stage / HBM_PEAK_GBS,
+            "plain_rhs": plain,
             "launches_by_kind": {k: timing[m][0] for m, k in enumerate(KIND_NAMES)},
             "mean_ms_by_kind": {k: (timing[m][1] / timing[m][0] if timing[m][0] else None)
                                 for m, k in enumerate(KIND_NAMES)},
             "alg_bytes_per_node_by_kind": dict(zip(KIND_NAMES, ALG_BYTES_BY_KIND)),
-            "achieved_rhs_only_104B": achieved_104,
         },
         "cpu_baseline": cpu,
     }

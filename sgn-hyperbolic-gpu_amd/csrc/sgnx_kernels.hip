@@ -524,6 +524,10 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     yhJ[y & 1][side][ky] = jh;
   }
 
+  auto face_node_now = [&](int r) -> bool {
+    unsigned n;
+    return face_node(r, n);
+  };
   // the interface lanes' global operands, loaded one row ahead
   StageRaw<MODE> fR;
   double fH = 0.0;
@@ -559,6 +563,19 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     //   kCand: c0 = u, c1 = 2/9 kc + 1/3 k, c2 = 7/24 kc + 0.25 k
     //   kErr:  c0 = u0, c2 = z
     double c0[NV], c1[NV], c2[NV];
+    // the interface lane's global node state and face depth for this row
+    // (loaded together with, and just before, this row's node operands)
+    double SG[NV];
+    double Hf = 0.0;
+    const bool haveG = (isX || isY) && face_node_now(r);
+    if (isX || isY) {
+      Hf = fH;
+      if (haveG) {
+        stage_combine<MODE>(fR, sa.a, SG);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) SG[v] = SG[v] + 0.0;
+      }
+    }
     if (isNode) {
       double Un[NV];
       stage_combine<MODE>(pR, sa.a, Un);
@@ -593,6 +610,7 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     // them above those uses: the loads then land in the registers the loop
     // carries, so no copy (and no wait for them) is needed at the back-edge.
     __builtin_amdgcn_sched_barrier(0);
+    if (r + 1 < ey1) face_prefetch(r + 1);
     if (isNode && r + 1 < ey1) {
       const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
       stage_load<MODE>(P, gn, pR);
@@ -604,14 +622,10 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     // ---- B ---------------------------------------------------------------------
     if (isX || isY) {
       double W[NV];
-      unsigned gnode;
-      const bool haveG = face_node(r, gnode);
       if (haveG) {
-        stage_combine<MODE>(fR, sa.a, W);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) W[v] = W[v] + 0.0;
+        for (int v = 0; v < NV; ++v) W[v] = SG[v];
       }
-      const double Hf = fH;
       if (isX) {
         // own side: a node of this row, the halo node, or a ghost of the
         // other side's node (a wall mirrors it; the inflow side is the table)
@@ -663,9 +677,6 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
         for (int v = 0; v < NV; ++v) yJ[buf][side][v][ky] = jf[v];
         yhJ[buf][side][ky] = jh;
       }
-      // the next row's interface operands, after the last use of this row's
-      __builtin_amdgcn_sched_barrier(0);
-      if (r + 1 < ey1) face_prefetch(r + 1);
     }
     __syncthreads();
 
