@@ -160,6 +160,31 @@ int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows);
 int sgnx_enable_timing(sgnx_solver* s, int on);
 int sgnx_rhs_timing(sgnx_solver* s, long* launches, double* total_ms);
 
+/* ---- batches of independent runs (no counterpart in the reference, whose
+ * sweeps run as separate jobs: scripts/shoal_sweep.sbatch) -------------------
+ * Solvers of uneven sizes advanced together: each RHS launch of an adaptive
+ * step covers every running member (their row-sweep workgroups packed behind
+ * a per-member tile offset table), one launch per stage kind. Each member keeps
+ * its own buffers, time, step size, controller and tile decomposition, so its
+ * state and statistics are bit for bit those of sgnx_integrate on it alone.
+ * Members must use the row-sweep kernel on one device and outlive the batch,
+ * which runs on a stream of its own and waits for the members' streams first. */
+typedef struct sgnx_batch sgnx_batch;
+#define SGNX_BATCH_GRAPH 1   /* capture a step into a hipGraph, replay it while
+                                the set of running members is unchanged */
+int sgnx_batch_create(sgnx_solver* const* members, int n, sgnx_batch** out);
+int sgnx_batch_destroy(sgnx_batch* b);
+const char* sgnx_batch_last_error(const sgnx_batch* b);   /* b may be NULL after create */
+/* waves per SIMD the batched launches are built for, by launch kind (bit k set
+ * = three; kinds as sgnx_rhs_timing); defaults to the first member's setting */
+int sgnx_batch_set_waves(sgnx_batch* b, int waves3);
+/* dU[m] = R(U[m]) at t[m] for every member in one launch (device pointers) */
+int sgnx_batch_rhs(sgnx_batch* b, const double* const* U, double* const* dU, const double* t);
+/* sgnx_integrate(member m, t0[m], tEnd[m], dt0[m], &o[m], &st[m], NULL, 0) for
+ * every member, advanced in lockstep (members that finish drop out) */
+int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, const double* dt0,
+                         const sgnx_rk_opts* o, sgnx_rk_stats* st, int flags);
+
 /* ---- host setup (no device needed) ---------------------------------------- */
 
 /* Replaces sgn::FROperators1D(P, GaussLobatto) (src/core/fr_operators.cpp:134-200):

@@ -24,38 +24,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/sgnx.h"
-#include "sgnx_internal.h"
+#include "sgnx_handle.h"
 
-using namespace sgnx;
 
-struct sgnx_solver {
-  int P = 4, Nx = 0, Ny = 0;
-  double Lx = 0.0, Ly = 0.0, hx = 0.0, hy = 0.0;
-  Geometry g{};
-  Phys ph{};
-  hipStream_t stream = nullptr;
-  int device = 0;
-  // device buffers
-  OpTables* d_ops = nullptr;
-  double *d_Hnod = nullptr, *d_Hx = nullptr, *d_Hy = nullptr;
-  double *d_HfX = nullptr, *d_HfY = nullptr, *d_inflow = nullptr;
-  double *d_U = nullptr, *d_k1 = nullptr, *d_k2 = nullptr, *d_k3 = nullptr, *d_k4 = nullptr;
-  double *d_tmp = nullptr, *d_uNew = nullptr;
-  double *d_part = nullptr, *d_out = nullptr;
-  double* h_out = nullptr;   // pinned, 2 doubles
-  double inflowOmega = 0.0;
-  bool hasInflow = false;
-  std::string err;
-  // launch timing: event pairs around RHS launches while enabled, by stage mode
-  bool timing = false;
-  std::vector<hipEvent_t> events;   // 2 per recorded launch, reused
-  std::vector<int> timedMode;       // stage mode of each recorded launch
-  long timed = 0;
-  long long partCap = 0;            // doubles in d_part
-
-  size_t n() const { return (size_t)NV * (size_t)g.Nd; }
-};
 
 namespace {
 
@@ -86,14 +57,6 @@ void free_all(sgnx_solver* s) {
   s->events.clear();
 }
 
-GhostTime ghost_time(const sgnx_solver* s, double t) {
-  GhostTime gt{0.0, 0.0};
-  if (s->hasInflow) {
-    gt.s = std::sin(s->inflowOmega * t);
-    gt.c = std::cos(s->inflowOmega * t);
-  }
-  return gt;
-}
 
 // One RHS launch: plain input (mode kPlain, either kernel variant) or a fused
 // stage input (row sweep only), bracketed by timing events when enabled.
@@ -332,52 +295,24 @@ int sgnx_rhs_once(sgnx_solver* s, double* dU, size_t n, double t) {
 // in d_part.
 static int fused_bs3(sgnx_solver* s, double t, double dt, double atol, double rtol,
                      bool withErr) {
+  BS3Launch L[3];
+  bs3_launches(s, t, dt, atol, rtol, L);
   int rc;
-  Stage sa{};
-  sa.u = s->d_U;
-  sa.k1 = s->d_k1;
-  sa.a = 0.5 * dt;
-  if ((rc = rhs_launch(s, kAxpy, sa, s->d_k2, t + 0.5 * dt))) return rc;
-  sa.k1 = s->d_k2;
-  sa.a = 0.75 * dt;
-  sa.kc = s->d_k1;
-  sa.dt = dt;
-  sa.uNew = s->d_uNew;
-  sa.zOut = s->d_k3;
-  if ((rc = rhs_launch(s, kCand, sa, nullptr, t + 0.75 * dt))) return rc;
-  if (!withErr) return SGNX_OK;
-  Stage se{};
-  se.u = s->d_uNew;
-  se.u0 = s->d_U;
-  se.z = s->d_k3;
-  se.dt = dt;
-  se.atol = atol;
-  se.rtol = rtol;
-  se.part = s->d_part;
-  return rhs_launch(s, kErr, se, s->d_k4, t + dt);
+  for (int k = 0; k < (withErr ? 3 : 2); ++k)
+    if ((rc = rhs_launch(s, L[k].mode, L[k].sa, L[k].dU, L[k].t))) return rc;
+  return SGNX_OK;
 }
 
 int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgnx_rk_opts* o,
                    sgnx_rk_stats* st, double* step_log, long cap) {
   if (!s || !o || !st) return SGNX_EINVAL;
-  // rk.cpp:67-164
-  const int orderLow = 2;
-  const double expo = 1.0 / (orderLow + 1.0);
-  const double alpha = 0.7 * expo;
-  const double beta = 0.4 * expo;
-  sgnx_rk_stats S{};
-  S.tReached = t0;
   const size_t n = s->n();
-  double t = t0;
-  double dt = std::min(dt0, o->dtMax);
-  double errPrev = 1.0;
-  int rc = rhs_dev(s, s->d_U, s->d_k1, t);   // FSAL head
+  RkCtl c(o, t0, tEnd, dt0, n, step_log, cap);
+  int rc = rhs_dev(s, s->d_U, s->d_k1, c.t);   // FSAL head
   if (rc) return rc;
-  ++S.rhsEvals;
-  long attempt = 0;
-  while (t < tEnd - 1e-15 * std::max(1.0, std::abs(tEnd))) {
-    if (o->max_accepted > 0 && S.accepted >= o->max_accepted) break;
-    if (t + dt > tEnd) dt = tEnd - t;
+  ++c.S.rhsEvals;
+  while (c.next()) {
+    const double t = c.t, dt = c.dt;
     if (s->g.variant == 1) {
       // stage inputs, the candidate and the embedded error fused into the RHS
       if ((rc = fused_bs3(s, t, dt, o->atol, o->rtol, true))) return rc;
@@ -393,50 +328,18 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
       HIPCHK(s, launch_err_norm(s->d_U, s->d_uNew, s->d_k1, s->d_k2, s->d_k3, s->d_k4, dt,
                                 o->atol, o->rtol, n, s->d_part, s->d_out, s->stream));
     }
-    S.rhsEvals += 3;
+    c.S.rhsEvals += 3;
     HIPCHK(s, hipMemcpyAsync(s->h_out, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost,
                              s->stream));
     if ((rc = sync(s))) return rc;
-    const double errSum = s->h_out[0], errMax = s->h_out[1];
-    const double errL2 = std::sqrt(errSum / static_cast<double>(n));
-    const double errNorm = std::max(errL2, 1e-300);
-    if (step_log && attempt < cap) {
-      step_log[2 * attempt] = dt;
-      step_log[2 * attempt + 1] = errNorm;
-    }
-    ++attempt;
-    if (!(errNorm == errNorm)) {
-      // a non-finite error norm would make the controller's dt NaN and the loop
-      // endless; stop with the state at the last accepted step instead
-      S.aborted = 1;
-      S.dtLast = dt;
-      break;
-    }
-    if (errNorm <= 1.0) {
-      t += dt;
+    if (c.finish(s->h_out[0], s->h_out[1])) {
       std::swap(s->d_U, s->d_uNew);
       std::swap(s->d_k1, s->d_k4);
-      ++S.accepted;
-      S.tReached = t;
-      S.dtLast = dt;
-      S.maxNormLast = errMax;
-      errPrev = errNorm;
-    } else {
-      ++S.rejected;
     }
-    double fac = o->safety * std::pow(errNorm, -alpha) * std::pow(errPrev, beta);
-    fac = std::clamp(fac, o->facMin, o->facMax);
-    double dtNext = dt * fac;
-    dtNext = std::min(dtNext, o->dtMax);
-    if (dtNext < o->dtMin) {
-      S.aborted = 1;
-      S.dtLast = dtNext;
-      break;
-    }
-    dt = dtNext;
+    if (c.done) break;
   }
   if ((rc = sync(s))) return rc;
-  *st = S;
+  *st = c.S;
   return SGNX_OK;
 }
 

@@ -1,0 +1,303 @@
+// =============================================================================
+// sgnx_batch.hip -- many independent solvers of uneven sizes advanced together.
+//
+// Every RHS launch of an adaptive step covers all running members: the
+// row-sweep workgroups of every member are packed back to back (a per-member
+// tile offset table) into one launch per stage kind, followed by one launch
+// that folds each member's error partials. Each member keeps its own state
+// buffers, time, step size and controller (the same RkCtl as sgnx_integrate)
+// and its own tile decomposition, so its results are bit for bit those of
+// sgnx_integrate on it alone. The reference's counterpart is a parameter sweep
+// run as separate jobs (scripts/shoal_sweep.sbatch).
+//
+// A step is: member arguments (pinned host -> device), three stage launches,
+// the fold, the (sum, max) pairs (device -> pinned host), one synchronisation;
+// optionally captured once into a hipGraph and replayed while the set of
+// running members is unchanged (the copies read and write the same pinned
+// buffers, which the host refills before each replay).
+// =============================================================================
+
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sgnx_handle.h"
+
+struct sgnx_batch {
+  std::vector<sgnx_solver*> m;
+  int n = 0;
+  int device = 0;
+  int waves3 = 0x1;                    // bit k: launch kind k built for 3 waves/SIMD
+  hipStream_t stream = nullptr;
+  // argument tables, [4 launch kinds][n slots], and the tile offsets
+  BatchMember* h_mem = nullptr;        // pinned
+  BatchMember* d_mem = nullptr;
+  long long* h_ts = nullptr;           // pinned, n + 1
+  long long* d_ts = nullptr;
+  double* h_out = nullptr;             // pinned, 2 n
+  double* d_out = nullptr;
+  // the captured step and the running set it was captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int> graphSet;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+int bfail(sgnx_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  else g_batch_create_error = msg;
+  return code;
+}
+
+#define BCHK(b, call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return bfail((b), e_ == hipErrorOutOfMemory ? SGNX_ENOMEM : SGNX_EDEVICE,       \
+                   std::string(#call) + ": " + hipGetErrorString(e_));                \
+  } while (0)
+
+void drop_graph(sgnx_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graphSet.clear();
+}
+
+BatchMember member_args(const sgnx_solver* s, const double* U, double* dU, const Stage& sa,
+                        double t) {
+  BatchMember bm{};
+  bm.U = U;
+  bm.dU = dU;
+  bm.g = s->g;
+  bm.ph = s->ph;
+  bm.gt = ghost_time(s, t);
+  bm.sa = sa;
+  bm.nStrips = sweep_strips(s->g);
+  bm.nTiles = (int)sweep_tiles(s->g);
+  return bm;
+}
+
+// tile offsets of the members in `set` (slot order) into h_ts; returns the total
+long long fill_offsets(sgnx_batch* b, const std::vector<int>& set) {
+  long long acc = 0;
+  for (size_t i = 0; i < set.size(); ++i) {
+    b->h_ts[i] = acc;
+    acc += sweep_tiles(b->m[set[i]]->g);
+  }
+  b->h_ts[set.size()] = acc;
+  return acc;
+}
+
+// the device work of one attempted step for the members in `set` (slots
+// 0..A-1 of kinds 1..3 of h_mem filled by the caller)
+int enqueue_step(sgnx_batch* b, const std::vector<int>& set, long long total) {
+  const int A = (int)set.size();
+  const size_t n = (size_t)b->n;
+  BCHK(b, hipMemcpyAsync(b->d_mem, b->h_mem, 4 * n * sizeof(BatchMember),
+                         hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipMemcpyAsync(b->d_ts, b->h_ts, (n + 1) * sizeof(long long), hipMemcpyHostToDevice,
+                         b->stream));
+  for (int k = 1; k <= 3; ++k)
+    BCHK(b, launch_rhs_stage_batch(b->d_mem + k * n, b->d_ts, A, total, k,
+                                   (b->waves3 >> k) & 1, b->stream));
+  BCHK(b, launch_finish_batch(b->d_mem + 3 * n, A, b->d_out, b->stream));
+  BCHK(b, hipMemcpyAsync(b->h_out, b->d_out, 2 * (size_t)A * sizeof(double),
+                         hipMemcpyDeviceToHost, b->stream));
+  return SGNX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sgnx_batch_last_error(const sgnx_batch* b) {
+  return b ? b->err.c_str() : g_batch_create_error.c_str();
+}
+
+int sgnx_batch_create(sgnx_solver* const* members, int n, sgnx_batch** out) {
+  if (!out) return SGNX_EINVAL;
+  *out = nullptr;
+  if (!members || n < 1) return bfail(nullptr, SGNX_EINVAL, "batch: no members");
+  int dev = -1;
+  for (int i = 0; i < n; ++i) {
+    const sgnx_solver* s = members[i];
+    if (!s) return bfail(nullptr, SGNX_EINVAL, "batch: null member");
+    if (s->g.variant != 1)
+      return bfail(nullptr, SGNX_EINVAL, "batch: members must use the row-sweep kernel");
+    if (dev < 0) dev = s->device;
+    if (s->device != dev) return bfail(nullptr, SGNX_EINVAL, "batch: members on different devices");
+    for (int j = 0; j < i; ++j)
+      if (members[j] == s) return bfail(nullptr, SGNX_EINVAL, "batch: a member appears twice");
+  }
+  sgnx_batch* b = new (std::nothrow) sgnx_batch();
+  if (!b) return bfail(nullptr, SGNX_ENOMEM, "batch: out of host memory");
+  b->m.assign(members, members + n);
+  b->n = n;
+  b->device = dev;
+  int rc = SGNX_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == SGNX_OK) {
+      rc = (e == hipErrorOutOfMemory) ? SGNX_ENOMEM : SGNX_EDEVICE;
+      g_batch_create_error = std::string(what) + ": " + hipGetErrorString(e);
+    }
+  };
+  chk(hipSetDevice(dev), "hipSetDevice");
+  chk(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+  chk(hipHostMalloc((void**)&b->h_mem, 4 * (size_t)n * sizeof(BatchMember)), "hipHostMalloc");
+  chk(hipMalloc((void**)&b->d_mem, 4 * (size_t)n * sizeof(BatchMember)), "hipMalloc");
+  chk(hipHostMalloc((void**)&b->h_ts, ((size_t)n + 1) * sizeof(long long)), "hipHostMalloc");
+  chk(hipMalloc((void**)&b->d_ts, ((size_t)n + 1) * sizeof(long long)), "hipMalloc");
+  chk(hipHostMalloc((void**)&b->h_out, 2 * (size_t)n * sizeof(double)), "hipHostMalloc");
+  chk(hipMalloc((void**)&b->d_out, 2 * (size_t)n * sizeof(double)), "hipMalloc");
+  if (rc != SGNX_OK) {
+    sgnx_batch_destroy(b);
+    return rc;
+  }
+  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) b->waves3 = std::atoi(w) & 0xf;
+  else b->waves3 = members[0]->g.waves3;
+  *out = b;
+  return SGNX_OK;
+}
+
+int sgnx_batch_destroy(sgnx_batch* b) {
+  if (!b) return SGNX_EINVAL;
+  drop_graph(b);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->h_mem) (void)hipHostFree(b->h_mem);
+  if (b->h_ts) (void)hipHostFree(b->h_ts);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  for (void* p : {(void*)b->d_mem, (void*)b->d_ts, (void*)b->d_out})
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return SGNX_OK;
+}
+
+int sgnx_batch_set_waves(sgnx_batch* b, int waves3) {
+  if (!b || waves3 < 0 || waves3 > 0xf) return SGNX_EINVAL;
+  b->waves3 = waves3;
+  drop_graph(b);
+  return SGNX_OK;
+}
+
+int sgnx_batch_rhs(sgnx_batch* b, const double* const* U, double* const* dU, const double* t) {
+  if (!b || !U || !dU || !t) return SGNX_EINVAL;
+  const int n = b->n;
+  BCHK(b, hipSetDevice(b->device));
+  for (int i = 0; i < n; ++i) {
+    if (!U[i] || !dU[i] || U[i] == dU[i]) return bfail(b, SGNX_EINVAL, "batch rhs: bad buffers");
+    BCHK(b, hipStreamSynchronize(b->m[i]->stream));
+  }
+  std::vector<int> all(n);
+  for (int i = 0; i < n; ++i) {
+    all[i] = i;
+    Stage sa{};
+    sa.u = U[i];
+    b->h_mem[i] = member_args(b->m[i], U[i], dU[i], sa, t[i]);
+  }
+  const long long total = fill_offsets(b, all);
+  BCHK(b, hipMemcpyAsync(b->d_mem, b->h_mem, (size_t)n * sizeof(BatchMember),
+                         hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipMemcpyAsync(b->d_ts, b->h_ts, ((size_t)n + 1) * sizeof(long long),
+                         hipMemcpyHostToDevice, b->stream));
+  BCHK(b, launch_rhs_stage_batch(b->d_mem, b->d_ts, n, total, kPlain, b->waves3 & 1, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  return SGNX_OK;
+}
+
+int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, const double* dt0,
+                         const sgnx_rk_opts* opts, sgnx_rk_stats* stats, int flags) {
+  if (!b || !t0 || !tEnd || !dt0 || !opts || !stats) return SGNX_EINVAL;
+  const int n = b->n;
+  const bool useGraph = (flags & SGNX_BATCH_GRAPH) != 0;
+  BCHK(b, hipSetDevice(b->device));
+  for (int i = 0; i < n; ++i) BCHK(b, hipStreamSynchronize(b->m[i]->stream));
+  std::vector<RkCtl> ctl;
+  ctl.reserve(n);
+  for (int i = 0; i < n; ++i) ctl.emplace_back(&opts[i], t0[i], tEnd[i], dt0[i], b->m[i]->n(),
+                                               nullptr, 0);
+  // FSAL heads: k1 = R(U) of every member, one launch
+  {
+    std::vector<int> all(n);
+    for (int i = 0; i < n; ++i) {
+      all[i] = i;
+      sgnx_solver* s = b->m[i];
+      Stage sa{};
+      sa.u = s->d_U;
+      b->h_mem[i] = member_args(s, s->d_U, s->d_k1, sa, ctl[i].t);
+      ++ctl[i].S.rhsEvals;
+    }
+    const long long total = fill_offsets(b, all);
+    BCHK(b, hipMemcpyAsync(b->d_mem, b->h_mem, (size_t)n * sizeof(BatchMember),
+                           hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_ts, b->h_ts, ((size_t)n + 1) * sizeof(long long),
+                           hipMemcpyHostToDevice, b->stream));
+    BCHK(b, launch_rhs_stage_batch(b->d_mem, b->d_ts, n, total, kPlain, b->waves3 & 1,
+                                   b->stream));
+    // the copies above read the pinned tables asynchronously; the steps below
+    // refill them
+    BCHK(b, hipStreamSynchronize(b->stream));
+  }
+  std::vector<int> set;
+  int rc = SGNX_OK;
+  for (;;) {
+    set.clear();
+    for (int i = 0; i < n; ++i)
+      if (ctl[i].next()) set.push_back(i);
+    if (set.empty()) break;
+    const int A = (int)set.size();
+    for (int slot = 0; slot < A; ++slot) {
+      const int i = set[slot];
+      sgnx_solver* s = b->m[i];
+      BS3Launch L[3];
+      bs3_launches(s, ctl[i].t, ctl[i].dt, opts[i].atol, opts[i].rtol, L);
+      for (int k = 0; k < 3; ++k)
+        b->h_mem[(size_t)(k + 1) * n + slot] = member_args(s, L[k].sa.u, L[k].dU, L[k].sa, L[k].t);
+    }
+    const long long total = fill_offsets(b, set);
+    if (useGraph) {
+      if (!b->exec || b->graphSet != set) {
+        drop_graph(b);
+        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_step(b, set, total);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        BCHK(b, ec);
+        b->graph = g;
+        BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graphSet = set;
+      }
+      BCHK(b, hipGraphLaunch(b->exec, b->stream));
+    } else {
+      if ((rc = enqueue_step(b, set, total))) return rc;
+    }
+    BCHK(b, hipStreamSynchronize(b->stream));
+    for (int slot = 0; slot < A; ++slot) {
+      const int i = set[slot];
+      sgnx_solver* s = b->m[i];
+      ctl[i].S.rhsEvals += 3;
+      if (ctl[i].finish(b->h_out[2 * slot], b->h_out[2 * slot + 1])) {
+        std::swap(s->d_U, s->d_uNew);
+        std::swap(s->d_k1, s->d_k4);
+      }
+    }
+  }
+  BCHK(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i) stats[i] = ctl[i].S;
+  return SGNX_OK;
+}
+
+}  // extern "C"

@@ -78,6 +78,20 @@ struct Stage {
   double* part;       // kErr
 };
 
+// One grid of a batched launch: its operands, geometry, constants, inflow
+// time factors and stage, and its row-sweep decomposition (strips per chunk
+// row, tiles in all).
+struct BatchMember {
+  const double* U;
+  double* dU;
+  Geometry g;
+  Phys ph;
+  GhostTime gt;
+  Stage sa;
+  int nStrips;
+  int nTiles;
+};
+
 // ---- launchers (sgnx_kernels.hip); all asynchronous on `st` ----------------
 
 // dU = R(U)
@@ -90,6 +104,16 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
                             double* dU, GhostTime gt, hipStream_t st);
 // out[0] = sum, out[1] = max of nb (sum, carry, max) partial triples, fixed order
 hipError_t launch_finish(const double* part, long long nb, double* out, hipStream_t st);
+// The row sweep over every member of a batch in one launch (device arrays:
+// mem[nMem]; tileStart[m] = sum of the tiles of members before m, totalTiles
+// their sum); `mode` as launch_rhs_stage, for all members.
+hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileStart, int nMem,
+                                  long long totalTiles, int mode, bool waves3, hipStream_t st);
+// launch_finish for every member: member m's partials (mem[m].sa.part, its
+// nTiles) into out[2m], out[2m + 1]
+hipError_t launch_finish_batch(const BatchMember* mem, int nMem, double* out, hipStream_t st);
+// row-sweep strips per chunk row of a grid
+int sweep_strips(const Geometry& g);
 // workgroups of one row-sweep launch (sizes the partial buffer)
 long long sweep_tiles(const Geometry& g);
 // Hx, Hy = (D . Hnod) / J per element line (sgn2d.cpp:54-71)

@@ -384,10 +384,12 @@ __device__ __forceinline__ unsigned node_of(int Nx, int ex, int ey, int a, int b
   return ((unsigned)ey * (unsigned)Nx + (unsigned)ex) * NPP + a + NP * b;
 }
 
+// One workgroup's strip x chunk `tile` of one grid (the body of the plain and
+// the batched launches).
 template <int TW, int MODE, int WAVES>
-__global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
-k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph,
-            GhostTime gt, Stage sa, int nStrips) {
+__device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double* __restrict__ dU,
+                                           const Geometry& g, const Phys& ph, const GhostTime& gt,
+                                           const Stage& sa, int nStrips, int tile) {
   constexpr int NT = Sweep<TW>::NT, NXI = Sweep<TW>::NXI, NYI = Sweep<TW>::NYI;
   __shared__ double sU[NV][NT];
   __shared__ double sF[NV][NT];
@@ -404,7 +406,6 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
     sgR[threadIdx.x] = g.ops->gRp[threadIdx.x];
   }
 
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int chunk = tile / nStrips;
   const int ex0 = (tile - chunk * nStrips) * TW;
   const int nW = min(TW, g.Nx - ex0);
@@ -783,6 +784,32 @@ k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, P
   }
 }
 
+template <int TW, int MODE, int WAVES>
+__global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
+k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph,
+            GhostTime gt, Stage sa, int nStrips) {
+  sweep_tile<TW, MODE, WAVES>(U, dU, g, ph, gt, sa, nStrips, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// Many grids in one launch: the workgroups of every member's tiles, packed
+// back to back (tileStart[m] = first packed index of member m, ascending);
+// each member keeps its own tile decomposition, so its error partials, and
+// everything else, equal its own launch's.
+template <int TW, int MODE, int WAVES>
+__global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
+k_rhs_sweep_batch(const BatchMember* __restrict__ mem, const long long* __restrict__ tileStart,
+                  int nMem) {
+  const long long w = xcd_tile(blockIdx.x, gridDim.x);
+  int lo = 0, hi = nMem - 1;                  // last member with tileStart <= w
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tileStart[mid] <= w) lo = mid; else hi = mid - 1;
+  }
+  const BatchMember& M = mem[lo];
+  sweep_tile<TW, MODE, WAVES>(M.U, M.dU, M.g, M.ph, M.gt, M.sa, M.nStrips,
+                              (int)(w - tileStart[lo]));
+}
+
 // Hx, Hy at every node (sgn2d.cpp:54-71).
 __global__ void __launch_bounds__(kThreads)
 k_bathy_grad(const double* __restrict__ H, double* __restrict__ Hx, double* __restrict__ Hy,
@@ -881,8 +908,8 @@ k_err_norm(const double* __restrict__ u, const double* __restrict__ uNew,
 
 // Second level: one block folds the nb partial triples in a fixed order.
 // out[0] = sum, out[1] = max.
-__global__ void __launch_bounds__(kThreads)
-k_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
+__device__ __forceinline__ void finish_block(const double* __restrict__ part, int nb,
+                                             double* __restrict__ out) {
   __shared__ double ss[kThreads], sc[kThreads], sm[kThreads];
   double s = 0.0, c = 0.0, m = 0.0;
   for (int i = threadIdx.x; i < nb; i += kThreads) {
@@ -898,6 +925,18 @@ k_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
     out[0] = ss[0] + sc[0];
     out[1] = sm[0];
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  finish_block(part, nb, out);
+}
+
+// one block per member: member m's partials into out[2m], out[2m+1]
+__global__ void __launch_bounds__(kThreads)
+k_finish_batch(const BatchMember* __restrict__ mem, int nMem, double* __restrict__ out) {
+  const BatchMember& M = mem[blockIdx.x];
+  finish_block(M.sa.part, M.nTiles, out + 2 * blockIdx.x);
 }
 
 // Diagnostics over nodes: 0 = max wave speed (sgn2d.cpp:295-304), 1 = mass
@@ -977,6 +1016,29 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
   else if (mode == kCand) { if (w3) SGNX_SWEEP(kCand, 3); else SGNX_SWEEP(kCand, 2); }
   else { if (w3) SGNX_SWEEP(kErr, 3); else SGNX_SWEEP(kErr, 2); }
 #undef SGNX_SWEEP
+  return hipGetLastError();
+}
+
+int sweep_strips(const Geometry& g) { return (g.Nx + kTileW - 1) / kTileW; }
+
+hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileStart, int nMem,
+                                  long long totalTiles, int mode, bool waves3, hipStream_t st) {
+  if (nMem < 1 || totalTiles < 1 || totalTiles > 0x7fffffffLL) return hipErrorInvalidValue;
+  if (mode < kPlain || mode > kErr) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)totalTiles), block(Sweep<kTileW>::NT);
+#define SGNX_SWEEPB(M, W) \
+  hipLaunchKernelGGL((k_rhs_sweep_batch<kTileW, M, W>), grid, block, 0, st, mem, tileStart, nMem)
+  if (mode == kPlain) { if (waves3) SGNX_SWEEPB(kPlain, 3); else SGNX_SWEEPB(kPlain, 2); }
+  else if (mode == kAxpy) { if (waves3) SGNX_SWEEPB(kAxpy, 3); else SGNX_SWEEPB(kAxpy, 2); }
+  else if (mode == kCand) { if (waves3) SGNX_SWEEPB(kCand, 3); else SGNX_SWEEPB(kCand, 2); }
+  else { if (waves3) SGNX_SWEEPB(kErr, 3); else SGNX_SWEEPB(kErr, 2); }
+#undef SGNX_SWEEPB
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_batch(const BatchMember* mem, int nMem, double* out, hipStream_t st) {
+  if (nMem < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_finish_batch, dim3((unsigned)nMem), dim3(kThreads), 0, st, mem, nMem, out);
   return hipGetLastError();
 }
 

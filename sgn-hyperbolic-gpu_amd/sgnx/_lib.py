@@ -24,6 +24,7 @@ BC_INFLOW = 2
 
 MEM_HOST = 0
 MEM_DEVICE = 1
+BATCH_GRAPH = 1   # SGNX_BATCH_GRAPH
 
 _dp = C.POINTER(C.c_double)
 
@@ -96,6 +97,13 @@ SIGNATURES = [
     ("sgnx_set_rhs_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("sgnx_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("sgnx_rhs_timing", C.c_int, [C.c_void_p, C.POINTER(C.c_long), _dp]),
+    ("sgnx_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    ("sgnx_batch_destroy", C.c_int, [C.c_void_p]),
+    ("sgnx_batch_last_error", C.c_char_p, [C.c_void_p]),
+    ("sgnx_batch_set_waves", C.c_int, [C.c_void_p, C.c_int]),
+    ("sgnx_batch_rhs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _dp]),
+    ("sgnx_batch_integrate", C.c_int, [C.c_void_p, _dp, _dp, _dp, C.POINTER(RkOpts),
+                                       C.POINTER(RkStats), C.c_int]),
     ("sgnx_fr_operators", C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("sgnx_node_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]),
     ("sgnx_face_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
