@@ -122,6 +122,72 @@ def plain_rhs_rate(s, nd, n=20):
             "frac": gbs / HBM_PEAK_GBS}
 
 
+def batched_sweep(args):
+    """The batched shoal sweep: --batch-members grids of seeded uneven sizes
+    (Nx in [128, 768], Ny = Nx/2; cases.sweep_member), each advanced
+    --batch-steps accepted adaptive steps, first one at a time with the plain
+    solver, then all together in one Batch (one launch per stage kind, a
+    captured step graph); every member must end bit for bit where its solo run
+    did. The tile height is chosen once by the batch autotuner and used by
+    both runs."""
+    import hashlib
+    import sgnx
+    from sgnx import cases
+    t = time.perf_counter()
+    sizes = cases.sweep_sizes(args.batch_members, seed=args.seed)
+    members, U0s, dt0s = [], [], []
+    for i, nx in enumerate(sizes):
+        c = cases.sweep_member(args.seed * 1000 + i + 1, nx)
+        s = sgnx.SGN2D(**c.solver_kwargs())
+        s.upload(c.U0)
+        dt0s.append(s.cflTimeStep(None, 0.5))
+        members.append(s)
+        U0s.append(c.U0)
+    nodes = sum(s.nDof() for s in members)
+    log(f"[bench] batch: {len(members)} members, Nx {min(sizes)}..{max(sizes)}, "
+        f"{nodes / 1e6:.1f} M nodes, built in {time.perf_counter() - t:.1f}s")
+    b = sgnx.Batch(members)
+    rows, waves = b.autotune()
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=args.batch_steps)
+    one = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=1)
+    members[0].integrateAdaptive(0.0, 1e9, dt0s[0], one)       # warm both code paths
+    b.integrateAdaptive(0.0, 1e9, dt0s, one, graph=True)
+
+    def digest(s):
+        return hashlib.sha1(s.download().tobytes()).hexdigest()
+
+    seq_wall, seq_dof, solo = 0.0, 0.0, []
+    for s, U0, dt0 in zip(members, U0s, dt0s):
+        s.upload(U0)
+        t0 = time.perf_counter()
+        st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+        seq_wall += time.perf_counter() - t0
+        seq_dof += 5.0 * s.nDof() * st.rhsEvals
+        solo.append((digest(s), st.as_dict()))
+    for s, U0 in zip(members, U0s):
+        s.upload(U0)
+    t0 = time.perf_counter()
+    sts = b.integrateAdaptive(0.0, 1e9, dt0s, opts, graph=True)
+    bat_wall = time.perf_counter() - t0
+    bat_dof = sum(5.0 * s.nDof() * st.rhsEvals for s, st in zip(members, sts))
+    same = all(digest(s) == h and st.as_dict() == sd
+               for s, st, (h, sd) in zip(members, sts, solo))
+    b.close()
+    for s in members:
+        s.close()
+    out = {"members": len(members), "nx_min": min(sizes), "nx_max": max(sizes),
+           "nodes": nodes, "steps_per_member": args.batch_steps,
+           "autotune": {"rows": rows, "waves3": waves},
+           "sequential": {"value": seq_dof / seq_wall, "wall_s": seq_wall},
+           "batched": {"value": bat_dof / bat_wall, "wall_s": bat_wall, "graph": True},
+           "speedup": (bat_dof / bat_wall) / (seq_dof / seq_wall),
+           "members_bit_identical": bool(same), "unit": "DOF-RHS-evals/s"}
+    log(f"[bench] batch: sequential {seq_dof / seq_wall / 1e9:.2f} GDOF-RHS/s ({seq_wall:.2f}s), "
+        f"batched {bat_dof / bat_wall / 1e9:.2f} GDOF-RHS/s ({bat_wall:.2f}s), "
+        f"x{out['speedup']:.2f}, identical={same}, rows {rows} waves {waves:#x}")
+    return out
+
+
 def cpu_baseline(case, gpu_solver, dt0, args):
     """The oracle on one core: FSAL head + 2 accepted steps of the same workload."""
     import numpy as np
@@ -167,6 +233,10 @@ def main():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--batch-members", type=int, default=64,
+                    help="members of the batched sweep leg (0 = skip it)")
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="accepted steps per member in the batched sweep leg")
     ap.add_argument("--profile-rhs", type=int, default=0)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -241,6 +311,13 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(case, s, dt0, args)
     s.close()
+    batched = None
+    if args.batch_members > 0:
+        try:
+            batched = batched_sweep(args)
+        except Exception as e:                    # reported, not fatal to the main line
+            batched = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] batch leg failed: {batched['error']}")
 
     out = {
         "metric": METRIC,
@@ -284,6 +361,7 @@ def main():
             "alg_bytes_per_node_by_kind": dict(zip(KIND_NAMES, ALG_BYTES_BY_KIND)),
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

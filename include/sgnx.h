@@ -185,6 +185,19 @@ int sgnx_batch_rhs(sgnx_batch* b, const double* const* U, double* const* dU, con
 int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, const double* dt0,
                          const sgnx_rk_opts* o, sgnx_rk_stats* st, int flags);
 
+/* Autotuning (no counterpart in the reference): times every launch kind of an
+ * adaptive step on the resident state for each candidate tile height (element
+ * rows per workgroup: 2, 4, 8, 12, 16, 24, 32, up to Ny) and each build
+ * (two or three waves per SIMD), keeps the fastest, and sets it (row-sweep
+ * kernel; reps = launches timed per candidate). Only scratch stage buffers are
+ * written; U and k1 are untouched. The tile height fixes the summation order
+ * of the error norm, so a run is reproducible bit for bit under the same
+ * setting (a batch member and its solo run share it). rows_out / waves3_out
+ * (optional) receive the choice. The batch form picks one tile height for
+ * every member and the batch's waves. */
+int sgnx_autotune(sgnx_solver* s, int reps, int* rows_out, int* waves3_out);
+int sgnx_batch_autotune(sgnx_batch* b, int reps, int* rows_out, int* waves3_out);
+
 /* ---- host setup (no device needed) ---------------------------------------- */
 
 /* Replaces sgn::FROperators1D(P, GaussLobatto) (src/core/fr_operators.cpp:134-200):

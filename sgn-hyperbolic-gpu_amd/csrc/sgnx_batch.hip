@@ -115,6 +115,20 @@ int enqueue_step(sgnx_batch* b, const std::vector<int>& set, long long total) {
   return SGNX_OK;
 }
 
+// tile heights tried by the autotuners (rows of elements per workgroup)
+constexpr int kTuneRows[] = {2, 4, 8, 12, 16, 24, 32};
+
+// a step size for the probe launches: any finite value times the same work
+constexpr double kProbeDt = 1e-9;
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -297,6 +311,159 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
   }
   BCHK(b, hipStreamSynchronize(b->stream));
   for (int i = 0; i < n; ++i) stats[i] = ctl[i].S;
+  return SGNX_OK;
+}
+
+// ---- autotuning (no counterpart in the reference) --------------------------
+
+int sgnx_autotune(sgnx_solver* s, int reps, int* rows_out, int* waves3_out) {
+  if (!s || reps < 1) return SGNX_EINVAL;
+  if (hipSetDevice(s->device) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess)
+    return SGNX_EDEVICE;
+  EventPair ev;
+  if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess)
+    return SGNX_EDEVICE;
+  const Geometry keep = s->g;
+  s->g.variant = 1;
+  BS3Launch L[3];
+  // ms of `reps` launches of kind k (0: R(U) into k2's buffer; 1..3: a BS3
+  // stage at the probe step) with the kind built for three waves or two
+  auto timed = [&](int k, bool three, float* ms) -> hipError_t {
+    s->g.waves3 = three ? (1 << k) : 0;
+    Stage sa{};
+    sa.u = s->d_U;
+    const Stage& st = (k == 0) ? sa : L[k - 1].sa;
+    double* dU = (k == 0) ? s->d_k2 : L[k - 1].dU;
+    const GhostTime gt = ghost_time(s, 0.0);
+    hipError_t e = launch_rhs_stage(s->g, s->ph, k, st, dU, gt, s->stream);   // warm
+    if (e != hipSuccess) return e;
+    if ((e = hipEventRecord(ev.a, s->stream)) != hipSuccess) return e;
+    for (int r = 0; r < reps; ++r)
+      if ((e = launch_rhs_stage(s->g, s->ph, k, st, dU, gt, s->stream)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ev.b, s->stream)) != hipSuccess) return e;
+    if ((e = hipEventSynchronize(ev.b)) != hipSuccess) return e;
+    return hipEventElapsedTime(ms, ev.a, ev.b);
+  };
+  double best = 1e300;
+  int bestRows = keep.rows, bestMask = keep.waves3;
+  for (int rows : kTuneRows) {
+    if (rows > s->g.Ny && rows != kTuneRows[0]) continue;
+    s->g.rows = rows;
+    bs3_launches(s, 0.0, kProbeDt, 1.0, 0.0, L);
+    double tot = 0.0;
+    int mask = 0;
+    for (int k = 1; k <= 3; ++k) {
+      float t2 = 0.0f, t3 = 0.0f;
+      if (timed(k, false, &t2) != hipSuccess || timed(k, true, &t3) != hipSuccess) {
+        s->g = keep;
+        s->err = "autotune: probe launch failed";
+        return SGNX_EDEVICE;
+      }
+      tot += std::min(t2, t3);
+      if (t3 < t2) mask |= 1 << k;
+    }
+    if (tot < best) {
+      best = tot;
+      bestRows = rows;
+      bestMask = mask;
+    }
+  }
+  s->g.rows = bestRows;
+  float p2 = 0.0f, p3 = 0.0f;
+  if (timed(0, false, &p2) != hipSuccess || timed(0, true, &p3) != hipSuccess) {
+    s->g = keep;
+    s->err = "autotune: probe launch failed";
+    return SGNX_EDEVICE;
+  }
+  s->g.waves3 = bestMask | (p3 < p2 ? 1 : 0);
+  if (rows_out) *rows_out = s->g.rows;
+  if (waves3_out) *waves3_out = s->g.waves3;
+  return SGNX_OK;
+}
+
+int sgnx_batch_autotune(sgnx_batch* b, int reps, int* rows_out, int* waves3_out) {
+  if (!b || reps < 1) return SGNX_EINVAL;
+  const int n = b->n;
+  BCHK(b, hipSetDevice(b->device));
+  for (int i = 0; i < n; ++i) BCHK(b, hipStreamSynchronize(b->m[i]->stream));
+  drop_graph(b);
+  EventPair ev;
+  BCHK(b, hipEventCreate(&ev.a));
+  BCHK(b, hipEventCreate(&ev.b));
+  std::vector<int> all(n);
+  for (int i = 0; i < n; ++i) all[i] = i;
+  int maxNy = 1;
+  for (sgnx_solver* s : b->m) maxNy = std::max(maxNy, s->g.Ny);
+  const size_t nn = (size_t)n;
+  // fill the tables of every kind for the members' current tile height
+  auto fill = [&]() -> long long {
+    for (int i = 0; i < n; ++i) {
+      sgnx_solver* s = b->m[i];
+      BS3Launch L[3];
+      bs3_launches(s, 0.0, kProbeDt, 1.0, 0.0, L);
+      Stage sa{};
+      sa.u = s->d_U;
+      b->h_mem[i] = member_args(s, s->d_U, s->d_k2, sa, 0.0);
+      for (int k = 0; k < 3; ++k)
+        b->h_mem[(size_t)(k + 1) * nn + i] = member_args(s, L[k].sa.u, L[k].dU, L[k].sa, L[k].t);
+    }
+    return fill_offsets(b, all);
+  };
+  auto timed = [&](int k, bool three, long long total, float* ms) -> hipError_t {
+    hipError_t e = launch_rhs_stage_batch(b->d_mem + k * nn, b->d_ts, n, total, k, three,
+                                          b->stream);
+    if (e != hipSuccess) return e;
+    if ((e = hipEventRecord(ev.a, b->stream)) != hipSuccess) return e;
+    for (int r = 0; r < reps; ++r)
+      if ((e = launch_rhs_stage_batch(b->d_mem + k * nn, b->d_ts, n, total, k, three,
+                                      b->stream)) != hipSuccess)
+        return e;
+    if ((e = hipEventRecord(ev.b, b->stream)) != hipSuccess) return e;
+    if ((e = hipEventSynchronize(ev.b)) != hipSuccess) return e;
+    return hipEventElapsedTime(ms, ev.a, ev.b);
+  };
+  std::vector<int> keepRows(n);
+  for (int i = 0; i < n; ++i) keepRows[i] = b->m[i]->g.rows;
+  double best = 1e300;
+  int bestRows = keepRows[0], bestMask = b->waves3;
+  for (int rows : kTuneRows) {
+    if (rows > maxNy && rows != kTuneRows[0]) continue;
+    for (sgnx_solver* s : b->m) s->g.rows = rows;
+    const long long total = fill();
+    BCHK(b, hipMemcpyAsync(b->d_mem, b->h_mem, 4 * nn * sizeof(BatchMember),
+                           hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_ts, b->h_ts, (nn + 1) * sizeof(long long),
+                           hipMemcpyHostToDevice, b->stream));
+    double tot = 0.0;
+    int mask = 0;
+    for (int k = 1; k <= 3; ++k) {
+      float t2 = 0.0f, t3 = 0.0f;
+      BCHK(b, timed(k, false, total, &t2));
+      BCHK(b, timed(k, true, total, &t3));
+      tot += std::min(t2, t3);
+      if (t3 < t2) mask |= 1 << k;
+    }
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (tot < best) {
+      best = tot;
+      bestRows = rows;
+      bestMask = mask;
+    }
+  }
+  for (sgnx_solver* s : b->m) s->g.rows = bestRows;
+  const long long total = fill();
+  BCHK(b, hipMemcpyAsync(b->d_mem, b->h_mem, 4 * nn * sizeof(BatchMember),
+                         hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipMemcpyAsync(b->d_ts, b->h_ts, (nn + 1) * sizeof(long long), hipMemcpyHostToDevice,
+                         b->stream));
+  float p2 = 0.0f, p3 = 0.0f;
+  BCHK(b, timed(0, false, total, &p2));
+  BCHK(b, timed(0, true, total, &p3));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  b->waves3 = bestMask | (p3 < p2 ? 1 : 0);
+  for (sgnx_solver* s : b->m) s->g.waves3 = b->waves3;
+  if (rows_out) *rows_out = bestRows;
+  if (waves3_out) *waves3_out = b->waves3;
   return SGNX_OK;
 }
 

@@ -104,6 +104,9 @@ SIGNATURES = [
     ("sgnx_batch_rhs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _dp]),
     ("sgnx_batch_integrate", C.c_int, [C.c_void_p, _dp, _dp, _dp, C.POINTER(RkOpts),
                                        C.POINTER(RkStats), C.c_int]),
+    ("sgnx_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("sgnx_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
     ("sgnx_fr_operators", C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("sgnx_node_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]),
     ("sgnx_face_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

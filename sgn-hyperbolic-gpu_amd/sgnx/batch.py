@@ -52,6 +52,13 @@ class Batch:
         """Waves per SIMD of the batched launches by kind (bit k set = three)."""
         self._check(self.lib.sgnx_batch_set_waves(self.h, int(mask)))
 
+    def autotune(self, reps=3):
+        """One tile height for every member and the batch's waves per SIMD, by
+        timing the batched launches; returns (rows, waves mask)."""
+        r, w = C.c_int(), C.c_int()
+        self._check(self.lib.sgnx_batch_autotune(self.h, int(reps), C.byref(r), C.byref(w)))
+        return r.value, w.value
+
     def rhs_device(self, U_ptrs, dU_ptrs, t=0.0):
         """dU[m] = R(U[m]) for every member in one launch (device addresses)."""
         n = len(self.members)

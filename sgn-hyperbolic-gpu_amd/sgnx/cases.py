@@ -225,6 +225,59 @@ def shoal(seed=0, Nx=1024, Ny=512, P=4, period=1.0, amp=0.01):
 
 
 # ---------------------------------------------------------------------------
+def point_noise(x, y, seed):
+    """A seeded value in [-1, 1) per physical point: every sample of one point
+    (the nodes of neighbouring elements, the face lines) gets the same value."""
+    ix = np.round(np.asarray(x, dtype=np.float64) * 1048576.0).astype(np.int64).view(np.uint64)
+    iy = np.round(np.asarray(y, dtype=np.float64) * 1048576.0).astype(np.int64).view(np.uint64)
+    z = (ix * np.uint64(0x9E3779B97F4A7C15)) ^ (iy * np.uint64(0xC2B2AE3D27D4EB4F)) \
+        ^ np.uint64((seed * 0x165667B19E3779F9) % (1 << 64))
+    # splitmix64 finaliser
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 / 9007199254740992.0) - 1.0
+
+
+def sweep_member(seed, Nx, P=4, period=1.0, amp=0.01):
+    """One member of the batched shoal sweep: the production workload's domain,
+    slope, inflow and walls at Nx x Nx/2 elements, a seeded round shoal (crest
+    ratio in [0.3, 0.9], radius in [1.5, 6] m) and seeded per-point relative
+    bathymetry noise of amplitude in [0, 2 %]."""
+    Ny = Nx // 2
+    Lx, Ly = 96.0, 48.0
+    h_deep = 0.45
+    par = Params(g=G, c=ALPHA * math.sqrt(G * h_deep), gamma=GAMMA)
+    rng = np.random.default_rng(seed)
+    sp = {"crest": float(rng.uniform(0.3, 0.9)), "r": float(rng.uniform(1.5, 6.0)),
+          "xc": float(rng.uniform(18.0, 30.0)), "yc": float(rng.uniform(14.0, 34.0)),
+          "noise": float(rng.uniform(0.0, 0.02))}
+
+    def depth_fn(x, y):
+        base = slope_depth(x)
+        local = slope_depth(np.full_like(x, sp["xc"]))
+        r2 = ((x - sp["xc"]) ** 2 + (y - sp["yc"]) ** 2) / sp["r"] ** 2
+        H = base - sp["crest"] * local * np.exp(-r2)
+        return H * (1.0 + sp["noise"] * point_noise(x, y, seed))
+
+    bc = ("inflow", "wall", "wall", "wall")
+    depth = depth_arrays(P, Nx, Ny, Lx, Ly, depth_fn, bc)
+    HfX = depth["HfaceX"].reshape(Ny, Nx + 1, P + 1)[:, 0, :].reshape(-1)
+    table, omega = inflow_table(Ny, P, HfX, period, amp)
+    Hn = depth["Hnod"]
+    z = np.zeros_like(Hn)
+    U0 = prim_to_state(Hn.copy(), z, z, z, z)
+    return Case(f"sweep{seed}", P, Nx, Ny, Lx, Ly, par, depth, bc, U0, inflow=(table, omega),
+                meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "seed": seed, "shoal": sp})
+
+
+def sweep_sizes(n=64, seed=0, nx_min=128, nx_max=768):
+    """Seeded element widths of the sweep's members (even, Ny = Nx / 2)."""
+    rng = np.random.default_rng(seed)
+    return [int(v) * 2 for v in rng.integers(nx_min // 2, nx_max // 2 + 1, size=n)]
+
+
+# ---------------------------------------------------------------------------
 def random_state(nd, seed, h0=1.0, amp=1.0):
     """Node-wise random state with h in h0*[0.6, 1.4] (exercises every branch
     of the arithmetic; not a smooth field)."""

@@ -271,6 +271,13 @@ class SGN2D:
         """0: one workgroup per 10-element row tile; 1: row sweep of `rows` rows."""
         L.check(self.lib.sgnx_set_rhs_kernel(self.h, int(variant), int(rows)), self.h)
 
+    def autotune(self, reps=3):
+        """Tile height and waves per SIMD of the row sweep chosen by timing the
+        launches of an adaptive step on the resident state; returns them."""
+        r, w = C.c_int(), C.c_int()
+        L.check(self.lib.sgnx_autotune(self.h, int(reps), C.byref(r), C.byref(w)), self.h)
+        return r.value, w.value
+
     def enable_timing(self, on=True):
         L.check(self.lib.sgnx_enable_timing(self.h, int(on)), self.h)
 

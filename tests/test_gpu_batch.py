@@ -99,3 +99,35 @@ def test_batch_rejects_bad_members():
     with pytest.raises(sgnx.SgnxError, match="row-sweep"):
         sgnx.Batch([s])
     s.close()
+
+
+def test_autotune_leaves_state_and_members_stay_identical():
+    """The autotuners write only stage scratch buffers (U is unchanged), pick a
+    candidate tile height, and a batch under the tuned setting still matches
+    its members' solo runs bit for bit."""
+    import sgnx
+    cs = _members()
+    ss = _solvers(cs, 16)
+    for c, s in zip(cs, ss):
+        s.upload(c.U0)
+    rows, waves = ss[1].autotune(reps=2)
+    assert rows in (2, 4, 8, 12, 16, 24, 32) and 0 <= waves <= 0xf
+    assert_bitwise(ss[1].download(), cs[1].U0, "state after autotune")
+    b = sgnx.Batch(ss)
+    rows, waves = b.autotune(reps=2)
+    for c, s in zip(cs, ss):
+        assert_bitwise(s.download(), c.U0, f"{c.name} state after batch autotune")
+    opts = [sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=k) for k in (4, 7, 3, 6)]
+    dt0 = [s.cflTimeStep(None, 0.5) for s in ss]
+    solo = []
+    for c, s, o, d in zip(cs, ss, opts, dt0):
+        st = s.integrateAdaptive(0.0, 1e9, d, o)
+        solo.append((s.download(), st.as_dict()))
+        s.upload(c.U0)
+    sts = b.integrateAdaptive(0.0, 1e9, dt0, opts, graph=True)
+    for i, (s, st) in enumerate(zip(ss, sts)):
+        assert st.as_dict() == solo[i][1]
+        assert_bitwise(s.download(), solo[i][0], f"member {i} under rows {rows}")
+    b.close()
+    for s in ss:
+        s.close()
