@@ -160,6 +160,27 @@ int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows);
 int sgnx_enable_timing(sgnx_solver* s, int on);
 int sgnx_rhs_timing(sgnx_solver* s, long* launches, double* total_ms);
 
+/* ---- diagnostics of the resident state (the host loops of the reference's
+ * applications, on the device) ------------------------------------------------
+ * Max of eta = h - H over the nodes strictly inside the window (x0, x1) x
+ * (y0, y1) (pass +-INFINITY for no bound) and the lowest node index attaining
+ * it, which is what a sequential scan with a strict ">" finds
+ * (apps/shoal_focus_mpi.cu:196-231, apps/tsunami_margin_mpi.cu:255-290); no
+ * node in the window gives (-1e300, -1). */
+int sgnx_eta_max(sgnx_solver* s, double x0, double x1, double y0, double y1, double* eta,
+                 long long* node);
+/* eta = h - H at n nodes (the gauges of apps/tsunami_margin_mpi.cu:274-278). */
+int sgnx_gauges(sgnx_solver* s, const long long* nodes, int n, double* eta);
+/* Element means of eta, u, v, p, H as src/io/vtk2d.h:39-81 forms them (0.25 *
+ * sum over b, a of qw[a] qw[b] f): out[f * Nx * Ny + ey * Nx + ex], 5 fields. */
+int sgnx_element_means(sgnx_solver* s, double* out);
+/* Gauges recorded by sgnx_integrate (and batches) after every accepted step:
+ * eta at n nodes, up to cap rows (set_gauges clears the log; n = 0 stops it).
+ * gauge_log copies min(cap, rows recorded) rows of n values and, if t is
+ * given, the time of each row. */
+int sgnx_set_gauges(sgnx_solver* s, const long long* nodes, int n, long cap);
+int sgnx_gauge_log(sgnx_solver* s, double* eta, double* t, long cap, long* rows);
+
 /* ---- batches of independent runs (no counterpart in the reference, whose
  * sweeps run as separate jobs: scripts/shoal_sweep.sbatch) -------------------
  * Solvers of uneven sizes advanced together: each RHS launch of an adaptive

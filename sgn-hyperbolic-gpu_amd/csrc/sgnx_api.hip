@@ -50,7 +50,8 @@ void free_all(sgnx_solver* s) {
   for (void* p : {(void*)s->d_ops, (void*)s->d_Hnod, (void*)s->d_Hx, (void*)s->d_Hy,
                   (void*)s->d_HfX, (void*)s->d_HfY, (void*)s->d_inflow, (void*)s->d_U,
                   (void*)s->d_k1, (void*)s->d_k2, (void*)s->d_k3, (void*)s->d_k4,
-                  (void*)s->d_tmp, (void*)s->d_uNew, (void*)s->d_part, (void*)s->d_out})
+                  (void*)s->d_tmp, (void*)s->d_uNew, (void*)s->d_part, (void*)s->d_out,
+                  (void*)s->d_gnodes, (void*)s->d_glog})
     if (p) (void)hipFree(p);
   if (s->h_out) (void)hipHostFree(s->h_out);
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
@@ -335,6 +336,7 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
     if (c.finish(s->h_out[0], s->h_out[1])) {
       std::swap(s->d_U, s->d_uNew);
       std::swap(s->d_k1, s->d_k4);
+      HIPCHK(s, record_gauges(s, s->stream, c.t));
     }
     if (c.done) break;
   }
@@ -379,6 +381,110 @@ int sgnx_bs3_step(sgnx_solver* s, double t, double dt) {
   }
   std::swap(s->d_U, s->d_uNew);
   return sync(s);
+}
+
+int sgnx_eta_max(sgnx_solver* s, double x0, double x1, double y0, double y1, double* eta,
+                 long long* node) {
+  if (!s || !eta || !node) return SGNX_EINVAL;
+  const int np = s->P + 1;
+  std::vector<double> xg(np);
+  if (sgnx_fr_operators(s->P, xg.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) !=
+      SGNX_OK)
+    return fail(s, SGNX_EINVAL, "eta_max: operator tables");
+  EtaWindow win{x0, x1, y0, y1, {}, {}};
+  for (int a = 0; a < np; ++a) {
+    win.xo[a] = 0.5 * s->hx * (xg[a] + 1.0);      // the host's node coordinates
+    win.yo[a] = 0.5 * s->hy * (xg[a] + 1.0);
+  }
+  HIPCHK(s, launch_eta_max(s->g, s->d_U, s->hx, s->hy, win, s->d_part, s->stream));
+  std::vector<double> part(2 * kReduceBlocks);
+  HIPCHK(s, hipMemcpyAsync(part.data(), s->d_part, part.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s->stream));
+  int rc = sync(s);
+  if (rc) return rc;
+  double best = -1e300;
+  long long bi = -1;
+  for (int k = 0; k < kReduceBlocks; ++k) {
+    const double v = part[2 * k];
+    const long long i = (long long)part[2 * k + 1];
+    if (v > best || (v == best && i >= 0 && (bi < 0 || i < bi))) {
+      best = v;
+      bi = i;
+    }
+  }
+  *eta = best;
+  *node = bi;
+  return SGNX_OK;
+}
+
+int sgnx_gauges(sgnx_solver* s, const long long* nodes, int n, double* eta) {
+  if (!s || (n > 0 && (!nodes || !eta)) || n < 0) return SGNX_EINVAL;
+  if (n == 0) return SGNX_OK;
+  for (int k = 0; k < n; ++k)
+    if (nodes[k] < 0 || nodes[k] >= s->g.Nd) return fail(s, SGNX_EINVAL, "gauges: node out of range");
+  long long* dn = nullptr;
+  double* dv = nullptr;
+  HIPCHK(s, hipMalloc((void**)&dn, n * sizeof(long long)));
+  hipError_t e = hipMalloc((void**)&dv, n * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(dn, nodes, n * sizeof(long long), hipMemcpyHostToDevice, s->stream);
+  if (e == hipSuccess) e = launch_gather_eta(s->g, s->d_U, dn, n, dv, s->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(eta, dv, n * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(dn);
+  if (dv) (void)hipFree(dv);
+  HIPCHK(s, e);
+  return SGNX_OK;
+}
+
+int sgnx_element_means(sgnx_solver* s, double* out) {
+  if (!s || !out) return SGNX_EINVAL;
+  const size_t ne = (size_t)s->Nx * s->Ny;
+  double* d = nullptr;
+  HIPCHK(s, hipMalloc((void**)&d, 5 * ne * sizeof(double)));
+  hipError_t e = launch_elem_means(s->g, s->d_U, d, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d, 5 * ne * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(d);
+  HIPCHK(s, e);
+  return SGNX_OK;
+}
+
+int sgnx_set_gauges(sgnx_solver* s, const long long* nodes, int n, long cap) {
+  if (!s || n < 0 || cap < 0 || (n > 0 && !nodes)) return SGNX_EINVAL;
+  for (int k = 0; k < n; ++k)
+    if (nodes[k] < 0 || nodes[k] >= s->g.Nd) return fail(s, SGNX_EINVAL, "gauges: node out of range");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (s->d_gnodes) (void)hipFree(s->d_gnodes);
+  if (s->d_glog) (void)hipFree(s->d_glog);
+  s->d_gnodes = nullptr;
+  s->d_glog = nullptr;
+  s->nG = 0;
+  s->gcap = 0;
+  s->grows = 0;
+  s->gtimes.clear();
+  if (n == 0 || cap == 0) return SGNX_OK;
+  HIPCHK(s, hipMalloc((void**)&s->d_gnodes, n * sizeof(long long)));
+  HIPCHK(s, hipMalloc((void**)&s->d_glog, (size_t)n * cap * sizeof(double)));
+  HIPCHK(s, hipMemcpy(s->d_gnodes, nodes, n * sizeof(long long), hipMemcpyHostToDevice));
+  s->nG = n;
+  s->gcap = cap;
+  return SGNX_OK;
+}
+
+int sgnx_gauge_log(sgnx_solver* s, double* eta, double* t, long cap, long* rows) {
+  if (!s || !rows) return SGNX_EINVAL;
+  const long r = std::min(cap, s->grows);
+  if (r > 0) {
+    if (!eta) return SGNX_EINVAL;
+    HIPCHK(s, hipMemcpyAsync(eta, s->d_glog, (size_t)r * s->nG * sizeof(double),
+                             hipMemcpyDeviceToHost, s->stream));
+    int rc = sync(s);
+    if (rc) return rc;
+    if (t) std::copy(s->gtimes.begin(), s->gtimes.begin() + r, t);
+  }
+  *rows = r;
+  return SGNX_OK;
 }
 
 static int diag(sgnx_solver* s, int which, double* out) {

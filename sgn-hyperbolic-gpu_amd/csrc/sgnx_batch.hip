@@ -306,6 +306,7 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
       if (ctl[i].finish(b->h_out[2 * slot], b->h_out[2 * slot + 1])) {
         std::swap(s->d_U, s->d_uNew);
         std::swap(s->d_k1, s->d_k4);
+        BCHK(b, record_gauges(s, b->stream, ctl[i].t));
       }
     }
   }

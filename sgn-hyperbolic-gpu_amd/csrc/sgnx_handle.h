@@ -41,11 +41,29 @@ struct sgnx_solver {
   std::vector<int> timedMode;       // stage mode of each recorded launch
   long timed = 0;
   long long partCap = 0;            // doubles in d_part
+  // gauges recorded by sgnx_integrate after every accepted step
+  long long* d_gnodes = nullptr;
+  double* d_glog = nullptr;         // gcap rows of nG values
+  int nG = 0;
+  long gcap = 0, grows = 0;
+  std::vector<double> gtimes;       // time of every recorded row
 
   size_t n() const { return (size_t)NV * (size_t)g.Nd; }
 };
 
 // Inflow ghost time factors of one RHS evaluation at time t.
+// eta at the gauges of s into its next log row (after an accepted step at t)
+inline hipError_t record_gauges(sgnx_solver* s, hipStream_t st, double t) {
+  if (s->nG < 1 || s->grows >= s->gcap) return hipSuccess;
+  const hipError_t e = launch_gather_eta(s->g, s->d_U, s->d_gnodes, s->nG,
+                                         s->d_glog + (size_t)s->grows * s->nG, st);
+  if (e == hipSuccess) {
+    ++s->grows;
+    s->gtimes.push_back(t);
+  }
+  return e;
+}
+
 inline GhostTime ghost_time(const sgnx_solver* s, double t) {
   GhostTime gt{0.0, 0.0};
   if (s->hasInflow) {

@@ -139,6 +139,19 @@ hipError_t launch_err_norm(const double* u, const double* uNew, const double* k1
 // Diagnostics of one state: which = 0 max wave speed (bitwise), 1 mass, 2 energy.
 hipError_t launch_diag(const Geometry& g, const Phys& ph, const double* U, int which,
                        double* part, double* out, hipStream_t st);
+// eta = h - H diagnostics (apps/shoal_focus_mpi.cu:196-231,
+// apps/tsunami_margin_mpi.cu:255-278, src/io/vtk2d.h:39-81)
+struct EtaWindow {
+  double x0, x1, y0, y1;       // open window
+  double xo[NP], yo[NP];       // 0.5 h (xi_a + 1): the node offsets in an element
+};
+// kReduceBlocks (max, node) partial pairs into part
+hipError_t launch_eta_max(const Geometry& g, const double* U, double hx, double hy,
+                          const EtaWindow& w, double* part, hipStream_t st);
+hipError_t launch_gather_eta(const Geometry& g, const double* U, const long long* nodes, int n,
+                             double* out, hipStream_t st);
+// out[f * Nx * Ny + e], f = eta, u, v, p, H
+hipError_t launch_elem_means(const Geometry& g, const double* U, double* out, hipStream_t st);
 
 }  // namespace sgnx
 

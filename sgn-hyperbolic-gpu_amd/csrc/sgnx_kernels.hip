@@ -939,6 +939,92 @@ k_finish_batch(const BatchMember* __restrict__ mem, int nMem, double* __restrict
   finish_block(M.sa.part, M.nTiles, out + 2 * blockIdx.x);
 }
 
+// Max of eta = h - H over the nodes inside the open window (x0, x1) x (y0, y1)
+// and the lowest node index attaining it (the result of a sequential scan with
+// a strict ">", apps/shoal_focus_mpi.cu:199-204): per block, (value, index)
+// partials in part[2 b], part[2 b + 1]; empty: (-1e300, -1). Node coordinates
+// are formed as the host forms them, ex hx + xo[a] with xo[a] = 0.5 hx (xi_a + 1).
+__global__ void __launch_bounds__(kThreads)
+k_eta_max(const double* __restrict__ U, const double* __restrict__ Hn, Geometry g, double hx,
+          double hy, EtaWindow w, double* __restrict__ part) {
+  __shared__ double sv[kThreads];
+  __shared__ long long si[kThreads];
+  double best = -1e300;
+  long long bi = -1;
+  const long long Nd = g.Nd;
+  const long long stride = (long long)gridDim.x * kThreads;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < Nd; i += stride) {
+    const long long e = i / NPP;
+    const int loc = (int)(i - e * NPP);
+    const int b = loc / NP, a = loc - NP * b;
+    const int ey = (int)(e / g.Nx), ex = (int)(e - (long long)ey * g.Nx);
+    const double x = ex * hx + w.xo[a];
+    const double y = ey * hy + w.yo[b];
+    if (!(x > w.x0 && x < w.x1 && y > w.y0 && y < w.y1)) continue;
+    const double eta = U[i] - Hn[i];
+    if (eta > best) { best = eta; bi = i; }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int off = kThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      const double v2 = sv[threadIdx.x + off];
+      const long long i2 = si[threadIdx.x + off];
+      const double v1 = sv[threadIdx.x];
+      const long long i1 = si[threadIdx.x];
+      if (v2 > v1 || (v2 == v1 && i2 >= 0 && (i1 < 0 || i2 < i1))) {
+        sv[threadIdx.x] = v2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sv[0];
+    part[2 * blockIdx.x + 1] = (double)si[0];
+  }
+}
+
+// eta = h - H at n given nodes
+__global__ void __launch_bounds__(kThreads)
+k_gather_eta(const double* __restrict__ U, const double* __restrict__ Hn,
+             const long long* __restrict__ nodes, int n, double* __restrict__ out) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k < n) out[k] = U[nodes[k]] - Hn[nodes[k]];
+}
+
+// Element means of eta, u, v, p, H (src/io/vtk2d.h:39-44, :67-81): for each
+// field, 0.25 * sum over b, a of qw[a] qw[b] f at the element's nodes, in that
+// order; out[f * Ne + e].
+__global__ void __launch_bounds__(kThreads)
+k_elem_means(const double* __restrict__ U, const double* __restrict__ Hn,
+             const OpTables* __restrict__ T, long long Nd, long long Ne,
+             double* __restrict__ out) {
+  const long long e = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (e >= Ne) return;
+  double m[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    double s = 0.0;
+    for (int b = 0; b < NP; ++b)
+      for (int a = 0; a < NP; ++a) {
+        const long long i = e * NPP + a + NP * b;
+        const double h = U[i];
+        double v;
+        if (f == 0) v = h - Hn[i];
+        else if (f == 1) v = U[(long long)IHU * Nd + i] / h;
+        else if (f == 2) v = U[(long long)IHV * Nd + i] / h;
+        else if (f == 3) v = U[(long long)IHP * Nd + i] / h;
+        else v = Hn[i];
+        s += T->qw[a] * T->qw[b] * v;
+      }
+    m[f] = 0.25 * s;
+  }
+#pragma unroll
+  for (int f = 0; f < 5; ++f) out[f * Ne + e] = m[f];
+}
+
 // Diagnostics over nodes: 0 = max wave speed (sgn2d.cpp:295-304), 1 = mass
 // integrand qw_a qw_b h (:319-332), 2 = energy integrand (:334-350).
 __global__ void __launch_bounds__(kThreads)
@@ -1100,6 +1186,28 @@ hipError_t launch_err_norm(const double* u, const double* uNew, const double* k1
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, st, part, kReduceBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_eta_max(const Geometry& g, const double* U, double hx, double hy,
+                          const EtaWindow& w, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_eta_max, dim3(kReduceBlocks), dim3(kThreads), 0, st, U, g.Hnod, g, hx, hy,
+                     w, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_eta(const Geometry& g, const double* U, const long long* nodes, int n,
+                             double* out, hipStream_t st) {
+  if (n < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_eta, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, st, U,
+                     g.Hnod, nodes, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_elem_means(const Geometry& g, const double* U, double* out, hipStream_t st) {
+  const long long Ne = (long long)g.Nx * g.Ny;
+  hipLaunchKernelGGL(k_elem_means, dim3((unsigned)((Ne + kThreads - 1) / kThreads)),
+                     dim3(kThreads), 0, st, U, g.Hnod, g.ops, g.Nd, Ne, out);
   return hipGetLastError();
 }
 

@@ -97,6 +97,12 @@ SIGNATURES = [
     ("sgnx_set_rhs_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("sgnx_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("sgnx_rhs_timing", C.c_int, [C.c_void_p, C.POINTER(C.c_long), _dp]),
+    ("sgnx_eta_max", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _dp,
+                               C.POINTER(C.c_longlong)]),
+    ("sgnx_gauges", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int, _dp]),
+    ("sgnx_element_means", C.c_int, [C.c_void_p, _dp]),
+    ("sgnx_set_gauges", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int, C.c_long]),
+    ("sgnx_gauge_log", C.c_int, [C.c_void_p, _dp, _dp, C.c_long, C.POINTER(C.c_long)]),
     ("sgnx_batch_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
     ("sgnx_batch_destroy", C.c_int, [C.c_void_p]),
     ("sgnx_batch_last_error", C.c_char_p, [C.c_void_p]),

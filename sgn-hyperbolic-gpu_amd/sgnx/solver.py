@@ -271,6 +271,50 @@ class SGN2D:
         """0: one workgroup per 10-element row tile; 1: row sweep of `rows` rows."""
         L.check(self.lib.sgnx_set_rhs_kernel(self.h, int(variant), int(rows)), self.h)
 
+    # ---- diagnostics (eta = h - H) ---------------------------------------------
+    def etaMax(self, window=None):
+        """(max eta, node index) over the nodes strictly inside window =
+        (x0, x1, y0, y1) (None: all nodes); the lowest index among ties."""
+        x0, x1, y0, y1 = window if window is not None else (-np.inf, np.inf, -np.inf, np.inf)
+        v, i = C.c_double(), C.c_longlong()
+        L.check(self.lib.sgnx_eta_max(self.h, float(x0), float(x1), float(y0), float(y1),
+                                      C.byref(v), C.byref(i)), self.h)
+        return v.value, i.value
+
+    def gauges(self, nodes):
+        """eta at the given node indices of the resident state."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        out = np.zeros(nodes.size)
+        L.check(self.lib.sgnx_gauges(self.h, nodes.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                     int(nodes.size), _dp(out)), self.h)
+        return out
+
+    def elementMeans(self):
+        """Element means of eta, u, v, p, H: array (5, Ny, Nx)."""
+        out = np.zeros(5 * self.Nx * self.Ny)
+        L.check(self.lib.sgnx_element_means(self.h, _dp(out)), self.h)
+        return out.reshape(5, self.Ny, self.Nx)
+
+    def setGauges(self, nodes, cap):
+        """Record eta at `nodes` after every accepted step of integrateAdaptive
+        (up to cap rows; clears the log)."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int64)
+        L.check(self.lib.sgnx_set_gauges(self.h, nodes.ctypes.data_as(C.POINTER(C.c_longlong)),
+                                         int(nodes.size), int(cap)), self.h)
+        self._nG = int(nodes.size)
+        self._gcap = int(cap)
+
+    def gaugeLog(self):
+        """(times, eta[rows, n]) recorded since setGauges."""
+        n, cap = getattr(self, "_nG", 0), getattr(self, "_gcap", 0)
+        eta = np.zeros(max(n * cap, 1))
+        t = np.zeros(max(cap, 1))
+        rows = C.c_long()
+        L.check(self.lib.sgnx_gauge_log(self.h, _dp(eta), _dp(t), int(cap), C.byref(rows)),
+                self.h)
+        r = rows.value
+        return t[:r].copy(), eta[:r * n].reshape(r, n).copy()
+
     def autotune(self, reps=3):
         """Tile height and waves per SIMD of the row sweep chosen by timing the
         launches of an adaptive step on the resident state; returns them."""
