@@ -219,6 +219,42 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
 int sgnx_autotune(sgnx_solver* s, int reps, int* rows_out, int* waves3_out);
 int sgnx_batch_autotune(sgnx_batch* b, int reps, int* rows_out, int* waves3_out);
 
+/* ---- discrete adjoint (no counterpart in the reference, which has none) ----
+ * Exact reverse-mode derivatives of the discrete map, nManning = 0 only
+ * (SGNX_EUNSUPPORTED otherwise). theta is the nodal still depth: Hnod = theta,
+ * Hx, Hy = (D theta)/J per element line (sgn2d.cpp:54-71), and each interface
+ * line node's face depth the mean of the two edge values meeting there (the
+ * wrapped pair on a periodic axis; the one interior value on a non-periodic
+ * side). For a still depth sampled from a continuous function at the nodes
+ * this is the reference's sampling (sgn2d.cpp:75-83). */
+/* Sets the still depth from theta (Nd values; mem as sgnx_rhs). */
+int sgnx_set_depth(sgnx_solver* s, const double* theta, size_t n, int mem);
+/* Ubar = (dR/dU (U, t))^T W and, if theta_bar is given, theta_bar =
+ * (dR/dtheta)^T W (Nd values); n = 5*Nd; mem as sgnx_rhs (device: Ubar
+ * distinct from U and W). */
+int sgnx_rhs_vjp(sgnx_solver* s, const double* U, const double* W, double* Ubar,
+                 double* theta_bar, size_t n, double t, int mem);
+/* Tape of adaptive runs: between tape_begin and tape_end, sgnx_integrate
+ * records every accepted step (t, dt, gauge log row) and snapshots of step
+ * input states, at most max_snapshots held (every K-th step; K doubles when
+ * the budget is full). tape_info: steps recorded, stride K, snapshots held.
+ * tape_replay: the recorded steps (fixed t, dt) on the resident state, with
+ * gauge recording -- the map the adjoint differentiates. */
+int sgnx_tape_begin(sgnx_solver* s, long max_snapshots);
+int sgnx_tape_end(sgnx_solver* s);
+int sgnx_tape_info(const sgnx_solver* s, long* steps, long* stride, long* snapshots);
+int sgnx_tape_replay(sgnx_solver* s);
+/* the recorded steps' start times and sizes (up to cap of each); steps = count */
+int sgnx_tape_steps(const sgnx_solver* s, double* t, double* dt, long cap, long* steps);
+/* Reverse sweep of the taped run (its step sizes held fixed): with Ubar_final
+ * the cotangent of the final state (or NULL) and gauge_bar that of the gauge
+ * log (rows x gauges as sgnx_gauge_log, or NULL), U0bar = the cotangent of
+ * the initial state (5*Nd) and theta_bar (Nd, optional) that of the still
+ * depth. The states inside a snapshot segment are recomputed (device memory:
+ * snapshots + K - 1 states + 7 states of scratch). mem as sgnx_rhs. */
+int sgnx_adjoint(sgnx_solver* s, const double* Ubar_final, const double* gauge_bar,
+                 double* U0bar, double* theta_bar, int mem);
+
 /* ---- host setup (no device needed) ---------------------------------------- */
 
 /* Replaces sgn::FROperators1D(P, GaussLobatto) (src/core/fr_operators.cpp:134-200):

@@ -1,0 +1,49 @@
+// =============================================================================
+// sgnx_adjoint.h -- device side of the discrete adjoint: the arguments of the
+// RHS vector-Jacobian product and the launchers of sgnx_adjoint.hip.
+// =============================================================================
+#ifndef SGNX_ADJOINT_H
+#define SGNX_ADJOINT_H
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/sgnx.h"
+#include "sgnx_internal.h"
+
+namespace sgnx {
+
+// out = acc0 + acc1 + acc2 + (dR/dU (U))^T W,  W = ca*A + cb*B,
+// thetaBar += (dR/dtheta)^T W  (when not null).
+// A, B, acc* and out are 5*Nd arrays; any acc* or B may be null; out may be
+// the same array as A or an acc* (every node is read before it is written,
+// by the thread that writes it). coef: 120*Nx*Ny doubles of scratch.
+struct VjpArgs {
+  const double* U;
+  const double* A;
+  double ca;
+  const double* B;
+  double cb;
+  const double* acc0;
+  const double* acc1;
+  const double* acc2;
+  double* out;
+  double* thetaBar;
+  double* coef;
+};
+
+inline long long vjp_coef_doubles(const Geometry& g) { return 120LL * g.Nx * g.Ny; }
+
+hipError_t launch_rhs_vjp(const Geometry& g, const Phys& ph, const VjpArgs& va, GhostTime gt,
+                          hipStream_t st);
+// HfX, HfY from the nodal still depth theta: the mean of the two edge values
+// meeting at a line node (the wrapped pair on a periodic axis), the one
+// interior edge value on a non-periodic side.
+hipError_t launch_face_depth(const Geometry& g, const double* theta, double* HfX, double* HfY,
+                             hipStream_t st);
+// lam[nodes[k]] += Gb[k]; thetaBar[nodes[k]] -= Gb[k] (thetaBar may be null)
+hipError_t launch_gauge_bar(const double* Gb, const long long* nodes, int n, double* lam,
+                            double* thetaBar, hipStream_t st);
+
+}  // namespace sgnx
+
+#endif  // SGNX_ADJOINT_H

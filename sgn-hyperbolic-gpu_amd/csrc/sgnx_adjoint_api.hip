@@ -1,0 +1,411 @@
+// =============================================================================
+// sgnx_adjoint_api.hip -- C-ABI of the discrete adjoint: the RHS
+// vector-Jacobian product, the still depth from one nodal field, and the
+// reverse sweep over a recorded adaptive run. No counterpart in the reference.
+//
+// A tape records, while sgnx_integrate runs, every accepted step (t, dt, its
+// gauge log row) and snapshots of the step input states: every K-th step,
+// with K doubled (every other snapshot dropped) whenever the snapshot budget
+// is full, so at most `max_snapshots` states are held whatever the run's
+// length. The reverse sweep walks the snapshot segments backwards; inside a
+// segment it recomputes the states with the forward's own stage kernels
+// (bitwise the recorded trajectory) and then runs, per step n from the last,
+//   k1 = R(u), y2 = u + dt/2 k1, k2 = R(y2), y3 = u + 3dt/4 k2    (rk.cpp:99-108)
+//   y3b = J(y3)^T (4/9 dt lam)
+//   y2b = J(y2)^T (1/3 dt lam + 3/4 dt y3b)
+//   lam = lam + y3b + y2b + J(u)^T (2/9 dt lam + 1/2 dt y2b)
+// with the gauge cotangents added to lam (and subtracted from theta's, the
+// gauge value being h - theta) before the step that produced their row. The
+// step sizes are those of the recording (the controller's dependence of dt on
+// the state is not differentiated), which makes the result the exact gradient
+// of the replayed run, sgnx_tape_replay.
+// =============================================================================
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sgnx_adjoint.h"
+#include "sgnx_handle.h"
+
+struct AdjTape {
+  bool recording = false;
+  long maxSnap = 0;
+  long K = 1;                      // snapshot stride in steps
+  std::vector<double*> snap;       // snap[j] = input state of step j*K
+  std::vector<double*> spare;      // allocated, unused snapshot buffers
+  std::vector<double> t, dt;       // accepted steps
+  std::vector<long> row;           // gauge log row written after step n, or -1
+  std::vector<long long> gnodes;   // gauge nodes while recording
+  // reverse-sweep buffers
+  double *lam = nullptr, *y3b = nullptr, *y2b = nullptr, *coef = nullptr, *thb = nullptr;
+  double* gb = nullptr;            // gauge cotangents staged from the host
+  size_t gbCap = 0;
+  std::vector<double*> seg;        // recomputed states of one segment
+};
+
+namespace {
+
+int fail(sgnx_solver* s, int code, const std::string& msg) {
+  s->err = msg;
+  return code;
+}
+
+#define HIPCHK(s, call)                                                               \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail((s), e_ == hipErrorOutOfMemory ? SGNX_ENOMEM : SGNX_EDEVICE,        \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                 \
+  } while (0)
+
+int alloc_state(sgnx_solver* s, double** p, size_t doubles) {
+  if (*p) return SGNX_OK;
+  HIPCHK(s, hipMalloc((void**)p, doubles * sizeof(double)));
+  return SGNX_OK;
+}
+
+int check_adjoint_support(sgnx_solver* s) {
+  if (s->ph.nManning != 0.0)
+    return fail(s, SGNX_EUNSUPPORTED, "adjoint: the Manning friction term is not differentiated");
+  return SGNX_OK;
+}
+
+// one RHS vector-Jacobian product on the solver's stream
+int vjp(sgnx_solver* s, const VjpArgs& va, double t) {
+  HIPCHK(s, launch_rhs_vjp(s->g, s->ph, va, ghost_time(s, t), s->stream));
+  return SGNX_OK;
+}
+
+VjpArgs vjp_args(AdjTape* tp, const double* U, const double* A, double ca, const double* B,
+                 double cb, double* out, double* thetaBar) {
+  VjpArgs va{};
+  va.U = U;
+  va.A = A;
+  va.ca = ca;
+  va.B = B;
+  va.cb = cb;
+  va.out = out;
+  va.thetaBar = thetaBar;
+  va.coef = tp->coef;
+  return va;
+}
+
+// u -> uNew by one BS3 step of size dt with the forward's kernels (fused stage
+// launches on the row sweep, separate combinations otherwise): bitwise the
+// state the adaptive loop accepted. Uses d_k1, d_k2, d_k3, d_tmp.
+int bs3_fixed(sgnx_solver* s, const double* u, double* uNew, double t, double dt) {
+  const size_t n = s->n();
+  HIPCHK(s, launch_rhs(s->g, s->ph, u, s->d_k1, ghost_time(s, t), s->stream));
+  if (s->g.variant == 1) {
+    Stage a{};
+    a.u = u;
+    a.k1 = s->d_k1;
+    a.a = 0.5 * dt;
+    HIPCHK(s, launch_rhs_stage(s->g, s->ph, kAxpy, a, s->d_k2, ghost_time(s, t + 0.5 * dt),
+                               s->stream));
+    Stage b{};
+    b.u = u;
+    b.k1 = s->d_k2;
+    b.a = 0.75 * dt;
+    b.kc = s->d_k1;
+    b.dt = dt;
+    b.uNew = uNew;
+    b.zOut = s->d_k3;
+    HIPCHK(s, launch_rhs_stage(s->g, s->ph, kCand, b, nullptr, ghost_time(s, t + 0.75 * dt),
+                               s->stream));
+  } else {
+    HIPCHK(s, launch_comb1(s->d_tmp, u, 0.5 * dt, s->d_k1, n, s->stream));
+    HIPCHK(s, launch_rhs(s->g, s->ph, s->d_tmp, s->d_k2, ghost_time(s, t + 0.5 * dt), s->stream));
+    HIPCHK(s, launch_comb1(s->d_tmp, u, 0.75 * dt, s->d_k2, n, s->stream));
+    HIPCHK(s, launch_rhs(s->g, s->ph, s->d_tmp, s->d_k3, ghost_time(s, t + 0.75 * dt), s->stream));
+    HIPCHK(s, launch_bs3_update(uNew, u, dt, s->d_k1, s->d_k2, s->d_k3, n, s->stream));
+  }
+  return SGNX_OK;
+}
+
+// lam <- (d u_{n+1} / d u_n)^T lam for the step u -> u_{n+1} of size dt at t;
+// thetaBar += the still-depth part. Uses d_k1, d_k2, d_k3, d_tmp.
+int step_adjoint(sgnx_solver* s, AdjTape* tp, const double* u, double t, double dt,
+                 double* thetaBar) {
+  const size_t n = s->n();
+  double* y2 = s->d_tmp;
+  double* y3 = s->d_k3;
+  HIPCHK(s, launch_rhs(s->g, s->ph, u, s->d_k1, ghost_time(s, t), s->stream));
+  HIPCHK(s, launch_comb1(y2, u, 0.5 * dt, s->d_k1, n, s->stream));
+  HIPCHK(s, launch_rhs(s->g, s->ph, y2, s->d_k2, ghost_time(s, t + 0.5 * dt), s->stream));
+  HIPCHK(s, launch_comb1(y3, u, 0.75 * dt, s->d_k2, n, s->stream));
+  int rc;
+  // uNew = u + dt ((2/9 k1 + 1/3 k2) + 4/9 k3)   (rk.cpp:106-108)
+  VjpArgs a3 = vjp_args(tp, y3, tp->lam, dt * (4.0 / 9.0), nullptr, 0.0, tp->y3b, thetaBar);
+  if ((rc = vjp(s, a3, t + 0.75 * dt))) return rc;
+  VjpArgs a2 = vjp_args(tp, y2, tp->lam, dt * (1.0 / 3.0), tp->y3b, 0.75 * dt, tp->y2b, thetaBar);
+  if ((rc = vjp(s, a2, t + 0.5 * dt))) return rc;
+  VjpArgs a1 = vjp_args(tp, u, tp->lam, dt * (2.0 / 9.0), tp->y2b, 0.5 * dt, tp->lam, thetaBar);
+  a1.acc0 = tp->lam;
+  a1.acc1 = tp->y3b;
+  a1.acc2 = tp->y2b;
+  return vjp(s, a1, t);
+}
+
+int ensure_vjp_buffers(sgnx_solver* s, AdjTape* tp) {
+  int rc;
+  if ((rc = alloc_state(s, &tp->coef, (size_t)vjp_coef_doubles(s->g)))) return rc;
+  if ((rc = alloc_state(s, &tp->thb, (size_t)s->g.Nd))) return rc;
+  return SGNX_OK;
+}
+
+AdjTape* tape_of(sgnx_solver* s) {
+  if (!s->tape) s->tape = new (std::nothrow) AdjTape();
+  return s->tape;
+}
+
+void release_snapshots(AdjTape* tp) {
+  for (double* p : tp->snap) tp->spare.push_back(p);
+  tp->snap.clear();
+}
+
+}  // namespace
+
+bool tape_recording(const sgnx_solver* s) { return s->tape && s->tape->recording; }
+
+int tape_step_begin(sgnx_solver* s) {
+  AdjTape* tp = s->tape;
+  const long n = (long)tp->t.size();
+  if (n % tp->K != 0 || (long)tp->snap.size() > n / tp->K) return SGNX_OK;
+  while ((long)tp->snap.size() >= tp->maxSnap) {
+    // budget full: keep every other snapshot, double the stride
+    std::vector<double*> keep;
+    for (size_t j = 0; j < tp->snap.size(); ++j) {
+      if (j % 2 == 0) keep.push_back(tp->snap[j]);
+      else tp->spare.push_back(tp->snap[j]);
+    }
+    tp->snap.swap(keep);
+    tp->K *= 2;
+    if (n % tp->K != 0) return SGNX_OK;
+  }
+  double* p = nullptr;
+  if (!tp->spare.empty()) {
+    p = tp->spare.back();
+    tp->spare.pop_back();
+  } else {
+    HIPCHK(s, hipMalloc((void**)&p, s->n() * sizeof(double)));
+  }
+  tp->snap.push_back(p);
+  HIPCHK(s, hipMemcpyAsync(p, s->d_U, s->n() * sizeof(double), hipMemcpyDeviceToDevice,
+                           s->stream));
+  return SGNX_OK;
+}
+
+void tape_accept(sgnx_solver* s, double t, double dt, long row) {
+  AdjTape* tp = s->tape;
+  tp->t.push_back(t);
+  tp->dt.push_back(dt);
+  tp->row.push_back(row);
+}
+
+void tape_free(sgnx_solver* s) {
+  AdjTape* tp = s->tape;
+  if (!tp) return;
+  release_snapshots(tp);
+  for (double* p : tp->spare) (void)hipFree(p);
+  for (double* p : tp->seg) (void)hipFree(p);
+  for (double* p : {tp->lam, tp->y3b, tp->y2b, tp->coef, tp->thb, tp->gb})
+    if (p) (void)hipFree(p);
+  delete tp;
+  s->tape = nullptr;
+}
+
+extern "C" {
+
+int sgnx_set_depth(sgnx_solver* s, const double* theta, size_t n, int mem) {
+  if (!s || !theta) return SGNX_EINVAL;
+  if (n != (size_t)s->g.Nd) return fail(s, SGNX_EINVAL, "set_depth: size mismatch");
+  if (mem != SGNX_MEM_HOST && mem != SGNX_MEM_DEVICE)
+    return fail(s, SGNX_EINVAL, "set_depth: bad memory kind");
+  HIPCHK(s, hipMemcpyAsync(s->d_Hnod, theta, n * sizeof(double),
+                           mem == SGNX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
+                           s->stream));
+  HIPCHK(s, launch_bathy_grad(s->g, s->d_Hnod, s->d_Hx, s->d_Hy, s->stream));
+  HIPCHK(s, launch_face_depth(s->g, s->d_Hnod, s->d_HfX, s->d_HfY, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return SGNX_OK;
+}
+
+int sgnx_rhs_vjp(sgnx_solver* s, const double* U, const double* W, double* Ubar,
+                 double* theta_bar, size_t n, double t, int mem) {
+  if (!s || !U || !W || !Ubar) return SGNX_EINVAL;
+  if (n != s->n()) return fail(s, SGNX_EINVAL, "rhs_vjp: size mismatch");
+  if (mem != SGNX_MEM_HOST && mem != SGNX_MEM_DEVICE)
+    return fail(s, SGNX_EINVAL, "rhs_vjp: bad memory kind");
+  int rc;
+  if ((rc = check_adjoint_support(s))) return rc;
+  AdjTape* tp = tape_of(s);
+  if (!tp) return fail(s, SGNX_ENOMEM, "host allocation failed");
+  if ((rc = ensure_vjp_buffers(s, tp))) return rc;
+  const size_t Nd = (size_t)s->g.Nd;
+  const double *dU = U, *dW = W;
+  double* dOut = Ubar;
+  double* dTh = theta_bar;
+  if (mem == SGNX_MEM_HOST) {
+    HIPCHK(s, hipMemcpyAsync(s->d_tmp, U, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(s->d_k2, W, n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    dU = s->d_tmp;
+    dW = s->d_k2;
+    dOut = s->d_k3;
+    dTh = theta_bar ? tp->thb : nullptr;
+  } else if (Ubar == U || Ubar == W) {
+    return fail(s, SGNX_EINVAL, "rhs_vjp: Ubar must not overlap U or W");
+  }
+  if (dTh) HIPCHK(s, hipMemsetAsync(dTh, 0, Nd * sizeof(double), s->stream));
+  VjpArgs va = vjp_args(tp, dU, dW, 1.0, nullptr, 0.0, dOut, dTh);
+  if ((rc = vjp(s, va, t))) return rc;
+  if (mem == SGNX_MEM_HOST) {
+    HIPCHK(s, hipMemcpyAsync(Ubar, dOut, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    if (theta_bar)
+      HIPCHK(s, hipMemcpyAsync(theta_bar, dTh, Nd * sizeof(double), hipMemcpyDeviceToHost,
+                               s->stream));
+  }
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return SGNX_OK;
+}
+
+int sgnx_tape_begin(sgnx_solver* s, long max_snapshots) {
+  if (!s) return SGNX_EINVAL;
+  if (max_snapshots < 1) return fail(s, SGNX_EINVAL, "tape: max_snapshots must be >= 1");
+  AdjTape* tp = tape_of(s);
+  if (!tp) return fail(s, SGNX_ENOMEM, "host allocation failed");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  release_snapshots(tp);
+  // buffers beyond the new budget are returned to the device
+  while ((long)tp->spare.size() > max_snapshots) {
+    (void)hipFree(tp->spare.back());
+    tp->spare.pop_back();
+  }
+  tp->recording = true;
+  tp->maxSnap = max_snapshots;
+  tp->K = 1;
+  tp->t.clear();
+  tp->dt.clear();
+  tp->row.clear();
+  tp->gnodes = s->gnodes;
+  return SGNX_OK;
+}
+
+int sgnx_tape_end(sgnx_solver* s) {
+  if (!s) return SGNX_EINVAL;
+  if (s->tape) s->tape->recording = false;
+  return SGNX_OK;
+}
+
+int sgnx_tape_info(const sgnx_solver* s, long* steps, long* stride, long* snapshots) {
+  if (!s || !steps || !stride || !snapshots) return SGNX_EINVAL;
+  const AdjTape* tp = s->tape;
+  *steps = tp ? (long)tp->t.size() : 0;
+  *stride = tp ? tp->K : 1;
+  *snapshots = tp ? (long)tp->snap.size() : 0;
+  return SGNX_OK;
+}
+
+int sgnx_tape_steps(const sgnx_solver* s, double* t, double* dt, long cap, long* steps) {
+  if (!s || !steps || (cap > 0 && (!t || !dt))) return SGNX_EINVAL;
+  const AdjTape* tp = s->tape;
+  const long N = tp ? (long)tp->t.size() : 0;
+  for (long k = 0; k < std::min(N, cap); ++k) {
+    t[k] = tp->t[k];
+    dt[k] = tp->dt[k];
+  }
+  *steps = N;
+  return SGNX_OK;
+}
+
+int sgnx_tape_replay(sgnx_solver* s) {
+  if (!s) return SGNX_EINVAL;
+  AdjTape* tp = s->tape;
+  if (!tp || tp->t.empty()) return fail(s, SGNX_EINVAL, "tape_replay: no recorded steps");
+  if (tp->recording) return fail(s, SGNX_EINVAL, "tape_replay: the tape is still recording");
+  int rc;
+  for (size_t k = 0; k < tp->t.size(); ++k) {
+    if ((rc = bs3_fixed(s, s->d_U, s->d_uNew, tp->t[k], tp->dt[k]))) return rc;
+    std::swap(s->d_U, s->d_uNew);
+    HIPCHK(s, record_gauges(s, s->stream, tp->t[k] + tp->dt[k]));
+  }
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return SGNX_OK;
+}
+
+int sgnx_adjoint(sgnx_solver* s, const double* Ubar_final, const double* gauge_bar,
+                 double* U0bar, double* theta_bar, int mem) {
+  if (!s || !U0bar) return SGNX_EINVAL;
+  if (mem != SGNX_MEM_HOST && mem != SGNX_MEM_DEVICE)
+    return fail(s, SGNX_EINVAL, "adjoint: bad memory kind");
+  AdjTape* tp = s->tape;
+  if (!tp || tp->recording) return fail(s, SGNX_EINVAL, "adjoint: no finished tape");
+  int rc;
+  if ((rc = check_adjoint_support(s))) return rc;
+  const long N = (long)tp->t.size();
+  if ((long)tp->snap.size() < (N + tp->K - 1) / tp->K)
+    return fail(s, SGNX_EINVAL, "adjoint: the tape lacks snapshots");
+  if (gauge_bar && tp->gnodes != s->gnodes)
+    return fail(s, SGNX_EINVAL, "adjoint: the gauge nodes differ from those of the recording");
+  const size_t n = s->n(), Nd = (size_t)s->g.Nd;
+  if ((rc = ensure_vjp_buffers(s, tp))) return rc;
+  for (double** p : {&tp->lam, &tp->y3b, &tp->y2b})
+    if ((rc = alloc_state(s, p, n))) return rc;
+  const hipMemcpyKind in = mem == SGNX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  if (Ubar_final) HIPCHK(s, hipMemcpyAsync(tp->lam, Ubar_final, n * sizeof(double), in, s->stream));
+  else HIPCHK(s, hipMemsetAsync(tp->lam, 0, n * sizeof(double), s->stream));
+  HIPCHK(s, hipMemsetAsync(tp->thb, 0, Nd * sizeof(double), s->stream));
+  const double* gb = nullptr;
+  const size_t gdoubles = (size_t)s->grows * s->nG;
+  if (gauge_bar && gdoubles > 0) {
+    if (mem == SGNX_MEM_HOST) {
+      if (tp->gbCap < gdoubles) {
+        if (tp->gb) (void)hipFree(tp->gb);
+        tp->gb = nullptr;
+        tp->gbCap = 0;
+        HIPCHK(s, hipMalloc((void**)&tp->gb, gdoubles * sizeof(double)));
+        tp->gbCap = gdoubles;
+      }
+      HIPCHK(s, hipMemcpyAsync(tp->gb, gauge_bar, gdoubles * sizeof(double), in, s->stream));
+      gb = tp->gb;
+    } else {
+      gb = gauge_bar;
+    }
+  }
+  // segment buffers: states K-1 steps past a snapshot
+  const long K = tp->K;
+  while ((long)tp->seg.size() < std::min(K, std::max(N, 1L)) - 1) {
+    double* p = nullptr;
+    HIPCHK(s, hipMalloc((void**)&p, n * sizeof(double)));
+    tp->seg.push_back(p);
+  }
+  for (long j = (N + K - 1) / K - 1; j >= 0; --j) {
+    const long n0 = j * K, n1 = std::min(n0 + K, N);
+    // states of the segment: u_{n0} (snapshot), u_{n0+1..n1-1} recomputed
+    std::vector<const double*> us(n1 - n0);
+    us[0] = tp->snap[j];
+    for (long m = n0 + 1; m < n1; ++m) {
+      double* dst = tp->seg[m - n0 - 1];
+      if ((rc = bs3_fixed(s, us[m - n0 - 1], dst, tp->t[m - 1], tp->dt[m - 1]))) return rc;
+      us[m - n0] = dst;
+    }
+    for (long m = n1 - 1; m >= n0; --m) {
+      if (gb && tp->row[m] >= 0 && tp->row[m] < s->grows)
+        HIPCHK(s, launch_gauge_bar(gb + (size_t)tp->row[m] * s->nG, s->d_gnodes, s->nG, tp->lam,
+                                   tp->thb, s->stream));
+      if ((rc = step_adjoint(s, tp, us[m - n0], tp->t[m], tp->dt[m], tp->thb))) return rc;
+    }
+  }
+  const hipMemcpyKind outk = mem == SGNX_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIPCHK(s, hipMemcpyAsync(U0bar, tp->lam, n * sizeof(double), outk, s->stream));
+  if (theta_bar)
+    HIPCHK(s, hipMemcpyAsync(theta_bar, tp->thb, Nd * sizeof(double), outk, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return SGNX_OK;
+}
+
+}  // extern "C"

@@ -56,6 +56,7 @@ void free_all(sgnx_solver* s) {
   if (s->h_out) (void)hipHostFree(s->h_out);
   for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
   s->events.clear();
+  tape_free(s);
 }
 
 
@@ -312,8 +313,14 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
   int rc = rhs_dev(s, s->d_U, s->d_k1, c.t);   // FSAL head
   if (rc) return rc;
   ++c.S.rhsEvals;
+  const bool taping = tape_recording(s);
+  long stepBegun = -1;
   while (c.next()) {
     const double t = c.t, dt = c.dt;
+    if (taping && stepBegun != c.S.accepted) {
+      if ((rc = tape_step_begin(s))) return rc;
+      stepBegun = c.S.accepted;
+    }
     if (s->g.variant == 1) {
       // stage inputs, the candidate and the embedded error fused into the RHS
       if ((rc = fused_bs3(s, t, dt, o->atol, o->rtol, true))) return rc;
@@ -336,7 +343,9 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
     if (c.finish(s->h_out[0], s->h_out[1])) {
       std::swap(s->d_U, s->d_uNew);
       std::swap(s->d_k1, s->d_k4);
+      const long rows = s->grows;
       HIPCHK(s, record_gauges(s, s->stream, c.t));
+      if (taping) tape_accept(s, t, dt, s->grows > rows ? rows : -1);
     }
     if (c.done) break;
   }
@@ -463,7 +472,9 @@ int sgnx_set_gauges(sgnx_solver* s, const long long* nodes, int n, long cap) {
   s->gcap = 0;
   s->grows = 0;
   s->gtimes.clear();
+  s->gnodes.clear();
   if (n == 0 || cap == 0) return SGNX_OK;
+  s->gnodes.assign(nodes, nodes + n);
   HIPCHK(s, hipMalloc((void**)&s->d_gnodes, n * sizeof(long long)));
   HIPCHK(s, hipMalloc((void**)&s->d_glog, (size_t)n * cap * sizeof(double)));
   HIPCHK(s, hipMemcpy(s->d_gnodes, nodes, n * sizeof(long long), hipMemcpyHostToDevice));

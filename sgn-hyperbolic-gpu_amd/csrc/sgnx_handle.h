@@ -47,11 +47,22 @@ struct sgnx_solver {
   int nG = 0;
   long gcap = 0, grows = 0;
   std::vector<double> gtimes;       // time of every recorded row
+  std::vector<long long> gnodes;    // host copy of the gauge nodes
+  // adjoint tape (accepted steps and state snapshots of the recorded runs)
+  struct AdjTape* tape = nullptr;
 
   size_t n() const { return (size_t)NV * (size_t)g.Nd; }
 };
 
-// Inflow ghost time factors of one RHS evaluation at time t.
+// Adjoint tape hooks of the adaptive loop (sgnx_adjoint_api.hip): before the
+// first attempt of every step (snapshot of the step's input state when due),
+// after every accepted step (its time, step size and gauge log row), and the
+// release of the tape's buffers.
+bool tape_recording(const sgnx_solver* s);
+int tape_step_begin(sgnx_solver* s);
+void tape_accept(sgnx_solver* s, double t, double dt, long row);
+void tape_free(sgnx_solver* s);
+
 // eta at the gauges of s into its next log row (after an accepted step at t)
 inline hipError_t record_gauges(sgnx_solver* s, hipStream_t st, double t) {
   if (s->nG < 1 || s->grows >= s->gcap) return hipSuccess;

@@ -113,6 +113,17 @@ SIGNATURES = [
     ("sgnx_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sgnx_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),
+    ("sgnx_set_depth", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    ("sgnx_rhs_vjp", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_size_t, C.c_double, C.c_int]),
+    ("sgnx_tape_begin", C.c_int, [C.c_void_p, C.c_long]),
+    ("sgnx_tape_end", C.c_int, [C.c_void_p]),
+    ("sgnx_tape_info", C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long),
+                                 C.POINTER(C.c_long)]),
+    ("sgnx_tape_replay", C.c_int, [C.c_void_p]),
+    ("sgnx_tape_steps", C.c_int, [C.c_void_p, _dp, _dp, C.c_long, C.POINTER(C.c_long)]),
+    ("sgnx_adjoint", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_int]),
     ("sgnx_fr_operators", C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("sgnx_node_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]),
     ("sgnx_face_coords", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,

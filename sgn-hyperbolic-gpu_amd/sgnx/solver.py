@@ -322,6 +322,87 @@ class SGN2D:
         L.check(self.lib.sgnx_autotune(self.h, int(reps), C.byref(r), C.byref(w)), self.h)
         return r.value, w.value
 
+    # ---- discrete adjoint ------------------------------------------------------
+    def setDepth(self, theta, device=False):
+        """Still depth from the nodal field theta (Nd values): Hnod = theta,
+        its element-line gradients, and face depths = the mean of the two edge
+        values meeting at each interface line node (one interior value on a
+        non-periodic side). device=True: theta is a device address."""
+        if device:
+            ptr = int(theta)
+        else:
+            theta = np.ascontiguousarray(theta, dtype=np.float64)
+            if theta.size != self._Nd:
+                raise ValueError("setDepth: size mismatch")
+            ptr = theta.ctypes.data
+        L.check(self.lib.sgnx_set_depth(self.h, C.c_void_p(ptr), self._Nd,
+                                        L.MEM_DEVICE if device else L.MEM_HOST), self.h)
+
+    def rhs_vjp(self, U, W, t=0.0, theta=True):
+        """(Ubar, thetaBar) = ((dR/dU)^T W, (dR/dtheta)^T W) at (U, t)."""
+        U = self._vec(U)
+        W = self._vec(W)
+        Ub = np.empty_like(U)
+        tb = np.empty(self._Nd) if theta else None
+        L.check(self.lib.sgnx_rhs_vjp(self.h, U.ctypes.data, W.ctypes.data, Ub.ctypes.data,
+                                      tb.ctypes.data if theta else None, U.size, float(t),
+                                      L.MEM_HOST), self.h)
+        return (Ub, tb) if theta else Ub
+
+    def rhs_vjp_device(self, U_ptr, W_ptr, Ubar_ptr, theta_bar_ptr=None, t=0.0):
+        L.check(self.lib.sgnx_rhs_vjp(self.h, C.c_void_p(U_ptr), C.c_void_p(W_ptr),
+                                      C.c_void_p(Ubar_ptr),
+                                      C.c_void_p(theta_bar_ptr) if theta_bar_ptr else None,
+                                      self.nTotal(), float(t), L.MEM_DEVICE), self.h)
+
+    def tapeBegin(self, max_snapshots=1 << 20):
+        """Record the accepted steps and state snapshots of the following
+        integrateAdaptive calls (at most max_snapshots states held)."""
+        L.check(self.lib.sgnx_tape_begin(self.h, int(max_snapshots)), self.h)
+
+    def tapeEnd(self):
+        L.check(self.lib.sgnx_tape_end(self.h), self.h)
+
+    def tapeInfo(self):
+        """(steps recorded, snapshot stride, snapshots held)."""
+        a, b, c = C.c_long(), C.c_long(), C.c_long()
+        L.check(self.lib.sgnx_tape_info(self.h, C.byref(a), C.byref(b), C.byref(c)), self.h)
+        return a.value, b.value, c.value
+
+    def tapeSteps(self):
+        """(t, dt) of every recorded step."""
+        n = self.tapeInfo()[0]
+        t, dt = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        got = C.c_long()
+        L.check(self.lib.sgnx_tape_steps(self.h, _dp(t), _dp(dt), n, C.byref(got)), self.h)
+        return t[:n].copy(), dt[:n].copy()
+
+    def tapeReplay(self):
+        """The taped steps (same t, dt) from the resident state, with gauge recording."""
+        L.check(self.lib.sgnx_tape_replay(self.h), self.h)
+
+    def adjoint(self, Ubar_final=None, gauge_bar=None, theta=True):
+        """Cotangents (U0bar, thetaBar) of the taped run's initial state and still
+        depth for the final-state cotangent Ubar_final and the gauge-log
+        cotangent gauge_bar (rows x gauges, as gaugeLog)."""
+        Uf = self._vec(Ubar_final) if Ubar_final is not None else None
+        gb = None
+        if gauge_bar is not None:
+            gb = np.ascontiguousarray(gauge_bar, dtype=np.float64)
+        U0b = np.empty(self.nTotal())
+        tb = np.empty(self._Nd) if theta else None
+        L.check(self.lib.sgnx_adjoint(self.h, Uf.ctypes.data if Uf is not None else None,
+                                      gb.ctypes.data if gb is not None else None,
+                                      U0b.ctypes.data, tb.ctypes.data if theta else None,
+                                      L.MEM_HOST), self.h)
+        return (U0b, tb) if theta else U0b
+
+    def adjoint_device(self, Ubar_final_ptr, gauge_bar_ptr, U0bar_ptr, theta_bar_ptr=None):
+        """adjoint() on device addresses (0 / None for an absent input)."""
+        L.check(self.lib.sgnx_adjoint(self.h, C.c_void_p(Ubar_final_ptr or None),
+                                      C.c_void_p(gauge_bar_ptr or None), C.c_void_p(U0bar_ptr),
+                                      C.c_void_p(theta_bar_ptr or None), L.MEM_DEVICE), self.h)
+
     def enable_timing(self, on=True):
         L.check(self.lib.sgnx_enable_timing(self.h, int(on)), self.h)
 

@@ -1,0 +1,287 @@
+"""The discrete adjoint (sgnx_rhs_vjp, sgnx_adjoint) against central finite
+differences of the oracle.
+
+The oracle pins the forward map bit for bit (the GPU RHS and stages are
+bitwise equal to it), so a derivative of the GPU's map is pinned by finite
+differences of the oracle's: for a random direction d,
+    <Ubar, d>  vs  (L(x + eps d) - L(x - eps d)) / (2 eps)
+with L the scalar the cotangents define, and the same for single entries
+(corner, edge and interior nodes, every variable), so a dropped term, a wrong
+sign or a transposed operand shows. Central differences have error O(eps^2)
+plus rounding O(1e-16 / eps); the bar is a relative 1e-6.
+
+The still depth of these runs is one nodal field theta (face depths = the
+mean of the two edge values meeting at a line node, one interior value on a
+non-periodic side); `theta_depth` restates that rule in numpy, and the GPU's
+RHS under sgnx_set_depth(theta) is checked against the oracle built from it,
+bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from test_gpu_rhs import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-6
+TOL = 1e-6
+
+
+def theta_depth(th, Nx, Ny, bc):
+    T = np.asarray(th, dtype=np.float64).reshape(Ny, Nx, 5, 5)     # [ey, ex, b, a]
+    left, right = T[:, :, :, 0], T[:, :, :, 4]                     # [ey, ex, n]
+    bot, top = T[:, :, 0, :], T[:, :, 4, :]
+    fx = np.empty((Ny, Nx + 1, 5))
+    fx[:, 1:Nx] = 0.5 * (right[:, :-1] + left[:, 1:])
+    if bc[0] == "periodic":
+        fx[:, 0] = fx[:, Nx] = 0.5 * (right[:, -1] + left[:, 0])
+    else:
+        fx[:, 0], fx[:, Nx] = left[:, 0], right[:, -1]
+    fy = np.empty((Ny + 1, Nx, 5))
+    fy[1:Ny] = 0.5 * (top[:-1] + bot[1:])
+    if bc[2] == "periodic":
+        fy[0] = fy[Ny] = 0.5 * (top[-1] + bot[0])
+    else:
+        fy[0], fy[Ny] = bot[0], top[-1]
+    return {"Hnod": np.ascontiguousarray(th, dtype=np.float64), "HfaceX": fx.ravel(),
+            "HfaceY": fy.ravel()}
+
+
+def _cases():
+    from sgnx import cases
+    out = [cases.gate(Nx=8, Ny=6),
+           cases.gate(Nx=13, Ny=7, bc=("wall",) * 4),
+           cases.gate(Nx=11, Ny=5, bc=("periodic", "periodic", "wall", "wall")),
+           cases.shoal(seed=3, Nx=24, Ny=6)]
+    return out
+
+
+def _oracle(c, theta):
+    return OL.Oracle(c.P, c.Nx, c.Ny, c.Lx, c.Ly, c.par, theta_depth(theta, c.Nx, c.Ny, c.bc),
+                     c.bc, c.inflow)
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def _state(c, seed, amp=0.15):
+    from sgnx import cases
+    U = cases.random_state(c.nd, seed=seed, h0=0.6, amp=amp)
+    U[:c.nd] += c.depth["Hnod"] - 0.6
+    return U
+
+
+@pytest.mark.parametrize("ci", range(4))
+def test_set_depth_matches_oracle_bitwise(ci):
+    import sgnx
+    c = _cases()[ci]
+    rng = np.random.default_rng(ci)
+    theta = c.depth["Hnod"] + 0.01 * rng.uniform(-1, 1, c.nd)
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.setDepth(theta)
+    o = _oracle(c, theta)
+    U = _state(c, 40 + ci)
+    assert_bitwise(s.rhs(U, 0.3), o.rhs(U, 0.3), f"{c.name} rhs after set_depth")
+    Hx, Hy = s.stillDepthGrad()
+    _, oHx, oHy = o.bathymetry()
+    assert_bitwise(Hx, oHx, "Hx")
+    assert_bitwise(Hy, oHy, "Hy")
+    s.close()
+
+
+@pytest.mark.parametrize("ci", range(4))
+def test_rhs_vjp_matches_finite_differences(ci):
+    import sgnx
+    c = _cases()[ci]
+    rng = np.random.default_rng(100 + ci)
+    theta = c.depth["Hnod"] + 0.01 * rng.uniform(-1, 1, c.nd)
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.setDepth(theta)
+    U = _state(c, 50 + ci)
+    W = rng.normal(size=U.size)
+    t = 0.37
+    Ub, thb = s.rhs_vjp(U, W, t)
+    o = _oracle(c, theta)
+    dU = rng.uniform(-1.0, 1.0, U.size) * np.repeat([0.05, 0.05, 0.05, 0.02, 0.02], c.nd)
+    fd = W @ (o.rhs(U + EPS * dU, t) - o.rhs(U - EPS * dU, t)) / (2 * EPS)
+    assert _rel(Ub @ dU, fd) < TOL, (c.name, Ub @ dU, fd)
+    # per variable: each block of U is pinned on its own
+    for v in range(5):
+        d = np.zeros_like(dU)
+        d[v * c.nd:(v + 1) * c.nd] = dU[v * c.nd:(v + 1) * c.nd]
+        fdv = W @ (o.rhs(U + EPS * d, t) - o.rhs(U - EPS * d, t)) / (2 * EPS)
+        assert _rel(Ub @ d, fdv) < TOL, (c.name, v, Ub @ d, fdv)
+    dth = rng.uniform(-1.0, 1.0, c.nd) * 0.05
+    fdt = W @ (_oracle(c, theta + EPS * dth).rhs(U, t) -
+               _oracle(c, theta - EPS * dth).rhs(U, t)) / (2 * EPS)
+    assert _rel(thb @ dth, fdt) < TOL, (c.name, thb @ dth, fdt)
+    # the face depths cancel between each Rusanov flux and the element's own
+    # face flux (both evaluated at the same depth), so R does not depend on
+    # them: differences with the face depths frozen agree to rounding
+    def frozen_faces(th):
+        d = theta_depth(theta, c.Nx, c.Ny, c.bc)
+        d["Hnod"] = np.ascontiguousarray(th)
+        return OL.Oracle(c.P, c.Nx, c.Ny, c.Lx, c.Ly, c.par, d, c.bc, c.inflow)
+    fdx = W @ (frozen_faces(theta + EPS * dth).rhs(U, t) -
+               frozen_faces(theta - EPS * dth).rhs(U, t)) / (2 * EPS)
+    assert _rel(fdx, fdt) < 1e-8
+    # single entries: corner, edge and interior nodes of a boundary element and
+    # of an interior one, every variable
+    scaleU, scaleT = np.abs(Ub).max(), np.abs(thb).max()
+    for e in (0, c.Nx + 1, c.Nx * c.Ny - 1):
+        for loc in (0, 2, 12, 24):
+            i = 25 * e + loc
+            for v in range(5):
+                d = np.zeros_like(U)
+                d[v * c.nd + i] = 1.0
+                fdi = W @ (o.rhs(U + EPS * d, t) - o.rhs(U - EPS * d, t)) / (2 * EPS)
+                got = Ub[v * c.nd + i]
+                assert abs(got - fdi) <= TOL * abs(fdi) + 1e-9 * scaleU, (c.name, e, loc, v, got, fdi)
+            d = np.zeros(c.nd)
+            d[i] = 1.0
+            fdi = W @ (_oracle(c, theta + EPS * d).rhs(U, t) -
+                       _oracle(c, theta - EPS * d).rhs(U, t)) / (2 * EPS)
+            assert abs(thb[i] - fdi) <= TOL * abs(fdi) + 1e-9 * scaleT, (c.name, e, loc, thb[i], fdi)
+    s.close()
+
+
+def _taped_run(c, theta, U0, nodes, steps, max_snap):
+    import sgnx
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.setDepth(theta)
+    s.upload(U0)
+    s.setGauges(nodes, 4 * steps)
+    dt0 = s.cflTimeStep(None, 0.5)
+    s.tapeBegin(max_snap)
+    st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=steps))
+    s.tapeEnd()
+    assert st.accepted == steps
+    return s, s.download(), s.gaugeLog()[1]
+
+
+@pytest.mark.parametrize("ci", range(4))
+def test_adjoint_of_adaptive_run_matches_finite_differences(ci):
+    c = _cases()[ci]
+    rng = np.random.default_rng(200 + ci)
+    theta = c.depth["Hnod"] + 0.002 * rng.uniform(-1, 1, c.nd)
+    U0 = c.U0 + 1e-3 * rng.uniform(-1, 1, c.U0.size) * np.repeat([1, 0.5, 0.5, 0.2, 0.2], c.nd)
+    nodes = rng.choice(c.nd, size=7, replace=False).astype(np.int64)
+    steps = 6
+    s, UN, G = _taped_run(c, theta, U0, nodes, steps, 1 << 20)
+    Ubar = rng.normal(size=UN.size)
+    Gbar = rng.normal(size=G.shape)
+    U0b, thb = s.adjoint(Ubar, Gbar)
+    tt, dts = s.tapeSteps()
+    assert tt.size == steps
+    o0 = _oracle(c, theta)
+
+    def loss(U0x, thx, o=None):
+        o = o or _oracle(c, thx)
+        U = U0x.copy()
+        rows = []
+        for t, dt in zip(tt, dts):
+            U = o.bs3_step(U, t, dt)
+            rows.append(U[nodes] - thx[nodes])
+        return U, np.array(rows), Ubar @ U + np.sum(Gbar * np.array(rows))
+
+    # the oracle's replay of the taped steps is the recorded run, bit for bit
+    Ur, Gr, _ = loss(U0, theta, o0)
+    assert_bitwise(UN, Ur, f"{c.name}: oracle replay of the tape")
+    assert_bitwise(G, Gr, f"{c.name}: oracle replay gauges")
+    # and so is the device replay
+    s.upload(U0)
+    s.setGauges(nodes, 4 * steps)
+    s.tapeReplay()
+    assert_bitwise(s.download(), UN, f"{c.name}: device replay")
+    assert_bitwise(s.gaugeLog()[1], G, f"{c.name}: device replay gauges")
+
+    dU = rng.uniform(-1.0, 1.0, U0.size) * np.repeat([0.05, 0.05, 0.05, 0.02, 0.02], c.nd)
+    fd = (loss(U0 + EPS * dU, theta, o0)[2] - loss(U0 - EPS * dU, theta, o0)[2]) / (2 * EPS)
+    assert _rel(U0b @ dU, fd) < TOL, (c.name, U0b @ dU, fd)
+    dth = rng.uniform(-1.0, 1.0, c.nd) * 0.05
+    fdt = (loss(U0, theta + EPS * dth)[2] - loss(U0, theta - EPS * dth)[2]) / (2 * EPS)
+    assert _rel(thb @ dth, fdt) < TOL, (c.name, thb @ dth, fdt)
+    # the gauges' direct theta term (eta = h - theta) is part of it
+    direct = -np.bincount(nodes, weights=Gbar.sum(axis=0), minlength=c.nd)
+    assert _rel((thb - direct) @ dth, fdt) > 1e3 * TOL
+    s.close()
+
+
+def test_snapshot_thinning_gives_the_same_adjoint():
+    """A budget of 2 snapshots (stride grows to 8 over 13 steps, segments
+    recomputed) and one of 1 give the store-all cotangents bit for bit."""
+    c = _cases()[2]
+    rng = np.random.default_rng(7)
+    theta = c.depth["Hnod"].copy()
+    nodes = np.array([5, 77, 300], dtype=np.int64)
+    res = []
+    for budget in (1 << 20, 2, 1):
+        s, UN, G = _taped_run(c, theta, c.U0, nodes, 13, budget)
+        steps, stride, held = s.tapeInfo()
+        assert steps == 13 and held <= budget
+        if budget < 13:
+            assert stride > 1
+        r = np.random.default_rng(9)
+        res.append(s.adjoint(r.normal(size=UN.size), r.normal(size=G.shape)))
+        s.close()
+    for U0b, thb in res[1:]:
+        assert_bitwise(U0b, res[0][0], "U0bar under a snapshot budget")
+        assert_bitwise(thb, res[0][1], "thetaBar under a snapshot budget")
+    del rng
+
+
+def test_autograd_function_matches_finite_differences():
+    """torch.autograd over (eta0, theta) -> (final state, gauges), with
+    u0 = (theta + eta0, hu, hv, hw, hp) composed in torch: the gradients of a
+    loss of both outputs equal central differences of the taped step sequence
+    (replay), and a second backward through a replay gives the same gradients."""
+    torch = pytest.importorskip("torch")
+    import sgnx
+    from sgnx.autograd import replay, run_adaptive
+    c = _cases()[0]
+    rng = np.random.default_rng(11)
+    theta0 = c.depth["Hnod"].copy()
+    eta0 = c.U0[:c.nd] - theta0
+    rest = torch.tensor(c.U0[c.nd:].copy())
+    nodes = rng.choice(c.nd, size=5, replace=False).astype(np.int64)
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.setDepth(theta0)
+    s.upload(c.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=5)
+    wg = torch.tensor(rng.normal(size=(5, nodes.size)))
+
+    def loss(e, th, taped):
+        U0 = torch.cat([th + e, rest])
+        if taped:
+            UN, G = run_adaptive(s, U0, th, 0.0, 1e9, dt0, opts, nodes)
+        else:
+            UN, G = replay(s, U0, th, nodes)
+        return (wg * G).sum() + 0.5 * (UN[:c.nd] ** 2).sum()
+
+    def grads(taped):
+        e = torch.tensor(eta0, requires_grad=True)
+        th = torch.tensor(theta0, requires_grad=True)
+        L = loss(e, th, taped)
+        L.backward()
+        return L.item(), e.grad.numpy().copy(), th.grad.numpy().copy()
+
+    L0, ge, gt = grads(True)
+    assert s.tapeInfo()[0] == 5
+    L1, ge1, gt1 = grads(False)
+    assert L1 == L0
+    assert_bitwise(ge1, ge, "eta0 gradient through a replay")
+    assert_bitwise(gt1, gt, "theta gradient through a replay")
+    de = rng.uniform(-1, 1, c.nd) * 0.01
+    dth = rng.uniform(-1, 1, c.nd) * 0.01
+    with torch.no_grad():
+        T = lambda a: torch.tensor(a)
+        fde = (loss(T(eta0 + EPS * de), T(theta0), False).item() -
+               loss(T(eta0 - EPS * de), T(theta0), False).item()) / (2 * EPS)
+        fdt = (loss(T(eta0), T(theta0 + EPS * dth), False).item() -
+               loss(T(eta0), T(theta0 - EPS * dth), False).item()) / (2 * EPS)
+    assert _rel(ge @ de, fde) < TOL, (ge @ de, fde)
+    assert _rel(gt @ dth, fdt) < TOL, (gt @ dth, fdt)
+    s.close()
