@@ -422,18 +422,24 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
     const St sn = make_st(Un);
     PrimBar qn{0.0, 0.0, 0.0, 0.0};
     node_terms_acc(sn, Hl, Hxl, Hyl, hx, hy, ph, Wn, Ub, qn, Hlb, Hxb, Hyb);
-    double Fb[NV], Gb[NV];
+    // the nodal flux cotangents: D^T applied along the row (x) and the column
+    // (y) to the cotangents -W/Jx, -W/Jy of the two D-sums. The k loop stays
+    // rolled so that only one k's ten LDS operands are live at a time.
+    double Fb[NV] = {0.0, 0.0, 0.0, 0.0, 0.0}, Gb[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int k = 0; k < NP; ++k) {
+      const double dxk = sD[k * NP + a], dyk = sD[k * NP + b];
+      const int nx = jb + k + NP * b, ny = jb + a + NP * k;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        Fb[c] = Fb[c] + dxk * L.W[c][nx];
+        Gb[c] = Gb[c] + dyk * L.W[c][ny];
+      }
+    }
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-      double sf = 0.0, sg = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        sf = sf + sD[k * NP + a] * L.W[c][jb + k + NP * b];
-        sg = sg + sD[k * NP + b] * L.W[c][jb + a + NP * k];
-      }
-      // cotangents of the two D-sums: -W/Jx, -W/Jy
-      Fb[c] = -sf / g.Jx;
-      Gb[c] = -sg / g.Jy;
+      Fb[c] = -Fb[c] / g.Jx;
+      Gb[c] = -Gb[c] / g.Jy;
     }
     Hlb += flux_acc<0>(sn, Hl, ph, Fb, Ub, qn);
     Hlb += flux_acc<1>(sn, Hl, ph, Gb, Ub, qn);
