@@ -195,8 +195,8 @@ struct TileLds {
 // the FR lift's face jump) and lambda = g' . (c^2 W_hp vel / J) (that of the
 // central face depth of the non-conservative products), g' = gL' on a low
 // face, gR' on a high one; f = 0 left, 1 right, 2 bottom, 3 top, line node n
-__device__ __forceinline__ void face_coef(const TileLds& L, int jb, int f, int n, double Jx,
-                                          double Jy, double* Lam, double& lam) {
+__device__ __forceinline__ void face_coef(const TileLds& L, int jb, int f, int n, double iJx,
+                                          double iJy, double* Lam, double& lam) {
   const int dir = f >> 1;
   const bool hi = (f & 1) != 0;
   double sum[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -209,9 +209,9 @@ __device__ __forceinline__ void face_coef(const TileLds& L, int jb, int f, int n
     for (int c = 0; c < NV; ++c) sum[c] = sum[c] + gk * L.W[c][node];
     q = q + gk * (dir == 0 ? L.DX[node] : L.DY[node]);
   }
-  const double J = dir == 0 ? Jx : Jy;
+  const double iJ = dir == 0 ? iJx : iJy;
 #pragma unroll
-  for (int c = 0; c < NV; ++c) Lam[c] = -sum[c] / J;
+  for (int c = 0; c < NV; ++c) Lam[c] = -sum[c] * iJ;
   lam = q;
 }
 
@@ -236,8 +236,8 @@ __device__ __forceinline__ void tile_stage(const VjpArgs& va, const Geometry& g,
   const double h = va.U[i];
   if (sH) sH[t] = h;
   const double q = W[IHP] * ph.c2 / h;
-  L.DX[t] = q * va.U[(long long)IHU * Nd + i] / g.Jx;
-  L.DY[t] = q * va.U[(long long)IHV * Nd + i] / g.Jy;
+  L.DX[t] = q * va.U[(long long)IHU * Nd + i] * (1.0 / g.Jx);
+  L.DY[t] = q * va.U[(long long)IHV * Nd + i] * (1.0 / g.Jy);
 }
 
 // ---- launch 1: face-line cotangents -----------------------------------------------
@@ -259,7 +259,7 @@ k_vjp_coef(VjpArgs va, Geometry g, Phys ph) {
     const int jf = t / 20, r = t - 20 * jf;
     const int f = r / NP, n = r - NP * f;
     double Lam[NV], lam;
-    face_coef(L, NPP * jf, f, n, g.Jx, g.Jy, Lam, lam);
+    face_coef(L, NPP * jf, f, n, 1.0 / g.Jx, 1.0 / g.Jy, Lam, lam);
     double* cp = va.coef + (((long long)ey * Nx + ex0) * 20 + t) * 6;
 #pragma unroll
     for (int c = 0; c < NV; ++c) cp[c] = Lam[c];
@@ -285,46 +285,42 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
   const double* sD = L.D;
   const double* sgL = L.gL;
   const double* sgR = L.gR;
+  const double iJx = 1.0 / g.Jx, iJy = 1.0 / g.Jy;
 
   // phase 1: node loads, the cotangents of the two D-sums and of the depth
-  // gradients of the non-conservative products
+  // gradients of the non-conservative products; the face threads (20 per
+  // element, see phase 2) issue their global loads before the barrier too
   const int j = t / NPP, loc = t - NPP * (t / NPP);
   const int b = loc / NP, a = loc - NP * b;
   const bool isNode = j < nW;
   const long long i = ((long long)ey * Nx + ex0 + j) * NPP + loc;
-  tile_stage(va, g, ph, L, t, isNode, i, sH);
-  __syncthreads();
-
-  // phase 2: face nodes (20 per element): the interface lines each edge
-  // state feeds, their adjoint kept for the own side. A y line is handled in
-  // the x frame: with P the exchange of hu and hv, flux_y(U) = P flux_x(P U)
-  // and the y wave speed is the x one of P U, so its adjoint is P applied to
-  // the x adjoint of the exchanged states and cotangents.
-  if (t < 20 * nW) {
-    const int jf = t / 20, r = t - 20 * jf;
-    const int f = r / NP, n = r - NP * f;
-    const int ex = ex0 + jf;
-    const long long e = (long long)ey * Nx + ex;
-    const int dir = f >> 1;
-    const bool ownHigh = (f & 1) == 0;               // own element on the line's high side
-    const int pos = dir == 0 ? ex : ey, nL = dir == 0 ? Nx : Ny;
-    const int sideLo = dir == 0 ? 0 : 2;              // bc index of the low side
-    const bool periodic = g.bc[sideLo] == SGNX_BC_PERIODIC;
+  const bool isFace = t < 20 * nW;
+  const int jf = t / 20, r = t - 20 * (t / 20);
+  const int f = r / NP, n = r - NP * f;
+  const int ex = ex0 + jf;
+  const long long e = (long long)ey * Nx + ex;
+  const int dir = f >> 1;
+  const bool ownHigh = (f & 1) == 0;               // own element on the line's high side
+  const int pos = dir == 0 ? ex : ey, nL = dir == 0 ? Nx : Ny;
+  const int sideLo = dir == 0 ? 0 : 2;              // bc index of the low side
+  const bool periodic = g.bc[sideLo] == SGNX_BC_PERIODIC;
+  const int posO = ownHigh ? pos - 1 : pos + 1;
+  const bool inLine = posO >= 0 && posO < nL;
+  const int wrapO = posO < 0 ? nL - 1 : (posO >= nL ? 0 : posO);
+  const int bcSide = ownHigh ? sideLo : sideLo + 1;
+  const bool ghost = !inLine && !periodic;
+  const bool wall = ghost && g.bc[bcSide] == SGNX_BC_WALL;
+  // pass 0: the own line; pass 1 (an edge on the wrap of a periodic axis):
+  // the line at the far end, where only the far element's face is in line
+  const int passes = (!inLine && periodic) ? 2 : 1;
+  double Uo[NV], Uw[NV], Lw[NV], lw = 0.0, hfp[2] = {0.0, 0.0};
+  if (isFace) {
     const long long io = e * NPP + face_loc(f, n);
-    const int posO = ownHigh ? pos - 1 : pos + 1;
-    const bool inLine = posO >= 0 && posO < nL;
-    const int wrapO = posO < 0 ? nL - 1 : (posO >= nL ? 0 : posO);
-    const int bcSide = ownHigh ? sideLo : sideLo + 1;
-    const bool ghost = !inLine && !periodic;
-    const bool wall = ghost && g.bc[bcSide] == SGNX_BC_WALL;
-    double Uo[NV], Lo[NV], Uw[NV], Lw[NV], lo;
-    face_coef(L, NPP * jf, f, n, g.Jx, g.Jy, Lo, lo);
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
       Uo[c] = U[(long long)c * Nd + io];
       Lw[c] = 0.0;
     }
-    double lw = 0.0;
     if (!ghost) {
       // the element across the line (across the wrap on a periodic axis)
       const long long eo = dir == 0 ? (long long)ey * Nx + wrapO : (long long)wrapO * Nx + ex;
@@ -336,14 +332,44 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
         Lw[c] = cw[c];
       }
       lw = cw[NV];
-    } else if (wall) {
-#pragma unroll
-      for (int c = 0; c < NV; ++c) Uw[c] = Uo[c];
-      Uw[dir == 0 ? IHU : IHV] = -Uo[dir == 0 ? IHU : IHV];
-    } else {
+    } else if (!wall) {
       const double* cf = g.inflow + ((long long)ey * NP + n) * (NV * 3);
 #pragma unroll
       for (int c = 0; c < NV; ++c) Uw[c] = (cf[3 * c] + cf[3 * c + 1] * gt.s) + cf[3 * c + 2] * gt.c;
+    }
+#pragma unroll
+    for (int ps = 0; ps < 2; ++ps) {
+      if (ps < passes) {
+        const int line = ps == 0 ? pos + (ownHigh ? 0 : 1) : (ownHigh ? nL : 0);
+        hfp[ps] = dir == 0 ? g.HfX[((long long)ey * (Nx + 1) + line) * NP + n]
+                           : g.HfY[((long long)line * Nx + ex) * NP + n];
+      }
+    }
+  }
+  // the node's own operands of phase 3
+  double Un[NV], Hl = 0.0, Hxl = 0.0, Hyl = 0.0;
+  if (isNode) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) Un[c] = U[(long long)c * Nd + i];
+    Hl = g.Hnod[i];
+    Hxl = g.Hx[i];
+    Hyl = g.Hy[i];
+  }
+  tile_stage(va, g, ph, L, t, isNode, i, sH);
+  __syncthreads();
+
+  // phase 2: face nodes (20 per element): the interface lines each edge
+  // state feeds, their adjoint kept for the own side. A y line is handled in
+  // the x frame: with P the exchange of hu and hv, flux_y(U) = P flux_x(P U)
+  // and the y wave speed is the x one of P U, so its adjoint is P applied to
+  // the x adjoint of the exchanged states and cotangents.
+  if (isFace) {
+    double Lo[NV], lo;
+    face_coef(L, NPP * jf, f, n, iJx, iJy, Lo, lo);
+    if (wall) {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) Uw[c] = Uo[c];
+      Uw[dir == 0 ? IHU : IHV] = -Uo[dir == 0 ? IHU : IHV];
     }
     auto to_x = [&](double* x) {
       if (dir == 1) {
@@ -356,15 +382,10 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
     to_x(Lo);
     to_x(Uw);
     to_x(Lw);
-    // pass 0: the own line; pass 1 (an edge on the wrap of a periodic axis):
-    // the line at the far end, where only the far element's face is in line
-    const int passes = (!inLine && periodic) ? 2 : 1;
     double Ub[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double tf = 0.0;
     for (int ps = 0; ps < passes; ++ps) {
-      const int line = ps == 0 ? pos + (ownHigh ? 0 : 1) : (ownHigh ? nL : 0);
-      const double hf = dir == 0 ? g.HfX[((long long)ey * (Nx + 1) + line) * NP + n]
-                                 : g.HfY[((long long)line * Nx + ex) * NP + n];
+      const double hf = ps == 0 ? hfp[0] : hfp[1];
       const bool ownIn = ps == 0, othIn = ps == 1 || inLine;
       double UL[NV], UR[NV], LL[NV], LR[NV], UbL[NV], UbR[NV];
 #pragma unroll
@@ -410,15 +431,11 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
                  (sHS[fb + NP + b] - sH[jb + NP - 1 + NP * b]) * sgR[a]);
     dhy = dhy + ((sHS[fb + 2 * NP + a] - sH[jb + a]) * sgL[b] +
                  (sHS[fb + 3 * NP + a] - sH[jb + a + NP * (NP - 1)]) * sgR[b]);
-    const double hx = dhx / g.Jx, hy = dhy / g.Jy;
-    const double Hl = g.Hnod[i], Hxl = g.Hx[i], Hyl = g.Hy[i];
+    const double hx = dhx * iJx, hy = dhy * iJy;
     double Ub[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double Hlb, Un[NV], Wn[NV];
+    double Hlb, Wn[NV];
 #pragma unroll
-    for (int c = 0; c < NV; ++c) {
-      Un[c] = U[(long long)c * Nd + i];
-      Wn[c] = L.W[c][t];
-    }
+    for (int c = 0; c < NV; ++c) Wn[c] = L.W[c][t];
     const St sn = make_st(Un);
     PrimBar qn{0.0, 0.0, 0.0, 0.0};
     node_terms_acc(sn, Hl, Hxl, Hyl, hx, hy, ph, Wn, Ub, qn, Hlb, Hxb, Hyb);
@@ -438,8 +455,8 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
     }
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-      Fb[c] = -Fb[c] / g.Jx;
-      Gb[c] = -Gb[c] / g.Jy;
+      Fb[c] = -Fb[c] * iJx;
+      Gb[c] = -Gb[c] * iJy;
     }
     Hlb += flux_acc<0>(sn, Hl, ph, Fb, Ub, qn);
     Hlb += flux_acc<1>(sn, Hl, ph, Gb, Ub, qn);
@@ -478,8 +495,8 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
   // still depth: Hnod = theta, Hx = (D theta)/Jx, Hy = (D theta)/Jy per line
   __syncthreads();
   if (isNode) {
-    L.DX[t] = Hxb / g.Jx;
-    L.DY[t] = Hyb / g.Jy;
+    L.DX[t] = Hxb * iJx;
+    L.DY[t] = Hyb * iJy;
   }
   __syncthreads();
   if (isNode) {
