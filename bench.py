@@ -188,6 +188,90 @@ def batched_sweep(args):
     return out
 
 
+def adjoint_leg(args):
+    """Cost of a gradient on the timed workload: an adaptive run of
+    --adjoint-steps accepted steps from the shoal state, its still depth set
+    from the nodal field theta (SGN2D.setDepth), gauges at 16 nodes; then the
+    same run taped (every step's input state kept) and the reverse sweep for
+    random cotangents of the final state and of the gauge log, giving the
+    cotangents of the initial state and of theta. Reports the sweep and the
+    whole gradient (taped run + sweep) against the plain run, and one RHS
+    vector-Jacobian product against one RHS."""
+    import numpy as np
+    import sgnx
+    from sgnx import cases, hip
+    case = cases.shoal(seed=args.seed, Nx=args.nx, Ny=args.ny)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    s.setDepth(case.depth["Hnod"])
+    n, nd = case.U0.size, case.nd
+    rng = np.random.default_rng(args.seed)
+    nodes = np.linspace(0, nd - 1, 16).astype(np.int64)
+    steps = args.adjoint_steps
+    s.upload(case.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=steps)
+    Ubar = rng.normal(size=n)
+    Gbar = rng.normal(size=(steps, nodes.size))
+
+    def run(taped):
+        s.upload(case.U0)
+        s.setGauges(nodes, steps)
+        if taped:
+            s.tapeBegin(steps + 1)
+        t0 = time.perf_counter()
+        st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+        wall = time.perf_counter() - t0
+        if taped:
+            s.tapeEnd()
+        return st, wall
+
+    run(True)                                      # warm-up: allocates the tape
+    s.adjoint(Ubar, Gbar)
+    st0, t_fwd = run(False)
+    st1, t_tape = run(True)
+    Ud = hip.DeviceArray.from_host(Ubar)
+    Gd = hip.DeviceArray.from_host(Gbar)
+    U0b = hip.DeviceArray(n)
+    thb = hip.DeviceArray(nd)
+    t0 = time.perf_counter()
+    s.adjoint_device(Ud.address, Gd.address, U0b.address, thb.address)
+    t_adj = time.perf_counter() - t0
+    # one product against one RHS on the final state (device operands)
+    U = hip.DeviceArray.from_host(s.download())
+    W = hip.DeviceArray.from_host(rng.normal(size=n))
+    dU = hip.DeviceArray(n)
+    reps = 20
+    s.rhs_vjp_device(U.address, W.address, dU.address, thb.address)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.rhs_vjp_device(U.address, W.address, dU.address, thb.address)
+    t_vjp = (time.perf_counter() - t0) / reps
+    s.rhs_device(U.address, dU.address)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.rhs_device(U.address, dU.address)
+    hip.synchronize()
+    t_rhs = (time.perf_counter() - t0) / reps
+    ok = bool(np.all(np.isfinite(U0b.download())) and np.all(np.isfinite(thb.download())))
+    info = s.tapeInfo()
+    s.close()
+    out = {"workload": f"shoal {case.Nx}x{case.Ny}, {steps} accepted steps, 16 gauges",
+           "same_step_counts": st0.as_dict() == st1.as_dict(),
+           "accepted": st0.accepted, "rejected": st0.rejected,
+           "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "adjoint_ms": 1e3 * t_adj,
+           "adjoint_over_forward": t_adj / t_fwd,
+           "gradient_over_forward": (t_tape + t_adj) / t_fwd,
+           "snapshots": info[2], "stride": info[1],
+           "vjp_ms": 1e3 * t_vjp, "rhs_ms": 1e3 * t_rhs, "vjp_over_rhs": t_vjp / t_rhs,
+           "cotangents_finite": ok}
+    log(f"[bench] adjoint: forward {1e3 * t_fwd:.1f} ms, taped {1e3 * t_tape:.1f} ms, "
+        f"reverse sweep {1e3 * t_adj:.1f} ms -> sweep/forward {t_adj / t_fwd:.2f}, "
+        f"gradient/forward {(t_tape + t_adj) / t_fwd:.2f}; VJP {1e3 * t_vjp:.3f} ms vs RHS "
+        f"{1e3 * t_rhs:.3f} ms")
+    return out
+
+
 def cpu_baseline(case, gpu_solver, dt0, args):
     """The oracle on one core: FSAL head + 2 accepted steps of the same workload."""
     import numpy as np
@@ -237,6 +321,8 @@ def main():
                     help="members of the batched sweep leg (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="accepted steps per member in the batched sweep leg")
+    ap.add_argument("--adjoint-steps", type=int, default=20,
+                    help="accepted steps of the gradient leg (0 = skip)")
     ap.add_argument("--profile-rhs", type=int, default=0)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -319,6 +405,14 @@ def main():
             batched = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] batch leg failed: {batched['error']}")
 
+    adjoint = None
+    if args.adjoint_steps > 0:
+        try:
+            adjoint = adjoint_leg(args)
+        except Exception as e:                    # reported, not fatal to the main line
+            adjoint = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] adjoint leg failed: {adjoint['error']}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -362,6 +456,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

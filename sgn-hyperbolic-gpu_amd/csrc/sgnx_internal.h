@@ -25,6 +25,15 @@ struct OpTables {
   double qw[NP];
 };
 
+// Bijective remap of workgroup ids so that XCD x (ids x, x+8, x+16, ...)
+// receives the contiguous tile range [start_x, start_x + count_x).
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, r = nb - q * kXcd;
+  const int x = bid - (bid / kXcd) * kXcd, i = bid / kXcd;
+  return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // What one RHS launch needs besides the stage input and output.
 struct Geometry {
   int Nx, Ny;

@@ -42,15 +42,6 @@ struct Tile {
   static constexpr int NYI = NP * TW;                     // y-interface nodes per side
 };
 
-// Bijective remap of workgroup ids so that XCD x (ids x, x+8, x+16, ...)
-// receives the contiguous tile range [start_x, start_x + count_x).
-__device__ __forceinline__ int xcd_tile(int bid, int nb) {
-  constexpr int kXcd = 8;
-  const int q = nb / kXcd, r = nb - q * kXcd;
-  const int x = bid - (bid / kXcd) * kXcd, i = bid / kXcd;
-  return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
 // Node (a, b) of element (ex, ey), variable v (sgn2d.h:22-27, mesh2d.h:31).
 __device__ __forceinline__ long long nidx(const Geometry& g, int ex, int ey, int a, int b) {
   return ((long long)ey * g.Nx + ex) * NPP + a + NP * b;
