@@ -100,6 +100,37 @@ def profile_rhs(args):
     s.close()
 
 
+def profile_vjp(args):
+    """RHS vector-Jacobian products (k_vjp_coef + k_vjp_elem) on device-resident
+    buffers of the shoal state, with the theta cotangent, then one reverse sweep
+    of --profile-steps taped steps."""
+    import numpy as np
+    import sgnx
+    from sgnx import hip
+    case = build_case(args)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    s.setDepth(case.depth["Hnod"])
+    rng = np.random.default_rng(args.seed)
+    U = hip.DeviceArray.from_host(case.U0)
+    W = hip.DeviceArray.from_host(rng.normal(size=case.U0.size))
+    Ub = hip.DeviceArray(U.n)
+    tb = hip.DeviceArray(case.nd)
+    for _ in range(args.profile_vjp):
+        s.rhs_vjp_device(U.address, W.address, Ub.address, tb.address)
+    if args.profile_steps:
+        s.upload(case.U0)
+        dt0 = s.cflTimeStep(None, 0.5)
+        s.tapeBegin(args.profile_steps + 1)
+        st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=args.profile_steps))
+        s.tapeEnd()
+        U0b = hip.DeviceArray(U.n)
+        s.adjoint_device(W.address, 0, U0b.address, tb.address)
+        log(f"[bench] profile: {st.accepted} taped steps and their reverse sweep")
+    hip.synchronize()
+    log(f"[bench] profile: {args.profile_vjp} RHS products of {case.nd} nodes")
+    s.close()
+
+
 def plain_rhs_rate(s, nd, n=20):
     """The RHS kernel on its own (no stage operands): n launches R(U) on the
     final state of the timed run, device-resident, timed with HIP events."""
@@ -324,6 +355,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="accepted steps of the gradient leg (0 = skip)")
     ap.add_argument("--profile-rhs", type=int, default=0)
+    ap.add_argument("--profile-vjp", type=int, default=0,
+                    help="profile mode: this many RHS vector-Jacobian products")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="with --profile-rhs: also run this many adaptive steps")
@@ -337,6 +370,9 @@ def main():
         sys.exit(2)
     if args.profile_rhs:
         profile_rhs(args)
+        return
+    if args.profile_vjp:
+        profile_vjp(args)
         return
 
     import sgnx
