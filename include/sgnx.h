@@ -241,6 +241,13 @@ int sgnx_rhs_vjp(sgnx_solver* s, const double* U, const double* W, double* Ubar,
  * tape_replay: the recorded steps (fixed t, dt) on the resident state, with
  * gauge recording -- the map the adjoint differentiates. */
 int sgnx_tape_begin(sgnx_solver* s, long max_snapshots);
+/* States the reverse sweep may hold besides the snapshots while it recomputes
+ * a segment (-1, the default: all K - 1 of it). Fewer slots select a binomial
+ * schedule (Griewank's revolve bound) inside each segment, plus two states of
+ * scratch; the result is the same bit for bit. */
+int sgnx_tape_set_slots(sgnx_solver* s, long slots);
+/* of the last sgnx_adjoint: recorded steps it recomputed, slots it held at once */
+int sgnx_adjoint_stats(const sgnx_solver* s, long* recomputed_steps, long* peak_slots);
 int sgnx_tape_end(sgnx_solver* s);
 int sgnx_tape_info(const sgnx_solver* s, long* steps, long* stride, long* snapshots);
 int sgnx_tape_replay(sgnx_solver* s);
@@ -251,7 +258,8 @@ int sgnx_tape_steps(const sgnx_solver* s, double* t, double* dt, long cap, long*
  * log (rows x gauges as sgnx_gauge_log, or NULL), U0bar = the cotangent of
  * the initial state (5*Nd) and theta_bar (Nd, optional) that of the still
  * depth. The states inside a snapshot segment are recomputed (device memory:
- * snapshots + K - 1 states + 7 states of scratch). mem as sgnx_rhs. */
+ * snapshots + the slots of sgnx_tape_set_slots + 7 states of scratch). mem as
+ * sgnx_rhs. */
 int sgnx_adjoint(sgnx_solver* s, const double* Ubar_final, const double* gauge_bar,
                  double* U0bar, double* theta_bar, int mem);
 

@@ -9,7 +9,9 @@
 // is full, so at most `max_snapshots` states are held whatever the run's
 // length. The reverse sweep walks the snapshot segments backwards; inside a
 // segment it recomputes the states with the forward's own stage kernels
-// (bitwise the recorded trajectory) and then runs, per step n from the last,
+// (bitwise the recorded trajectory), holding at most `slots` of them at once
+// (a binomial schedule, Griewank's revolve bound, when that is fewer than the
+// segment's K - 1), and then runs, per step n from the last,
 //   k1 = R(u), y2 = u + dt/2 k1, k2 = R(y2), y3 = u + 3dt/4 k2    (rk.cpp:99-108)
 //   y3b = J(y3)^T (4/9 dt lam)
 //   y2b = J(y2)^T (1/3 dt lam + 3/4 dt y3b)
@@ -44,7 +46,11 @@ struct AdjTape {
   double *lam = nullptr, *y3b = nullptr, *y2b = nullptr, *coef = nullptr, *thb = nullptr;
   double* gb = nullptr;            // gauge cotangents staged from the host
   size_t gbCap = 0;
-  std::vector<double*> seg;        // recomputed states of one segment
+  long slots = -1;                 // states the sweep may hold besides the snapshots (-1: any)
+  std::vector<double*> seg;        // recomputation slots (allocated)
+  std::vector<double*> freeSeg;    // of those, the unused ones
+  double* pp[2] = {nullptr, nullptr};   // ping-pong pair for slot-free recomputation
+  long recomputed = 0, peakSlots = 0, inUse = 0;   // of the last sweep
 };
 
 namespace {
@@ -158,6 +164,125 @@ int ensure_vjp_buffers(sgnx_solver* s, AdjTape* tp) {
   return SGNX_OK;
 }
 
+// ---- recomputation inside a snapshot segment ------------------------------------
+// F(L, c): forward steps needed to reverse L steps from a held start state
+// with c free state slots, F(1, c) = 0, F(L, 0) = L (L - 1) / 2, and
+// F(L, c) = min_m m + F(L - m, c - 1) + F(m, c): Griewank's binomial bound
+// r L - binom(C + r, C + 1) with C = c + 1 checkpoints (the start state's
+// included) and r the least repetition number with binom(C + r, C) >= L.
+double binom_capped(long n, long k) {
+  double r = 1.0;
+  for (long i = 1; i <= k; ++i) {
+    r = r * (double)(n - k + i) / (double)i;
+    if (r > 1e30) return 1e30;
+  }
+  return r;
+}
+
+long fwd_cost(long L, long c) {
+  if (L <= 1) return 0;
+  if (c <= 0) return L * (L - 1) / 2;
+  const long C = c + 1;
+  long r = 1;
+  while (binom_capped(C + r, C) < (double)L) ++r;
+  return r * L - (long)binom_capped(C + r, C + 1);
+}
+
+// the first checkpoint of an optimal schedule: advance m steps and keep the state
+long best_split(long L, long c) {
+  long best = 1, cost = -1;
+  for (long m = 1; m < L; ++m) {
+    const long k = m + fwd_cost(L - m, c - 1) + fwd_cost(m, c);
+    if (cost < 0 || k < cost) {
+      cost = k;
+      best = m;
+    }
+  }
+  return best;
+}
+
+int take_slot(sgnx_solver* s, AdjTape* tp, double** out) {
+  if (tp->freeSeg.empty()) {
+    double* p = nullptr;
+    HIPCHK(s, hipMalloc((void**)&p, s->n() * sizeof(double)));
+    tp->seg.push_back(p);
+    tp->freeSeg.push_back(p);
+  }
+  *out = tp->freeSeg.back();
+  tp->freeSeg.pop_back();
+  tp->peakSlots = std::max(tp->peakSlots, ++tp->inUse);
+  return SGNX_OK;
+}
+
+void give_slot(AdjTape* tp, double* p) {
+  tp->freeSeg.push_back(p);
+  --tp->inUse;
+}
+
+// u_{a+m} from u (holding u_a) by m recorded steps, into dst (or, dst null,
+// into one of the ping-pong pair, returned in *out)
+int advance(sgnx_solver* s, AdjTape* tp, const double* u, long a, long m, double* dst,
+            const double** out) {
+  const double* src = u;
+  int rc;
+  for (long k = 0; k < m; ++k) {
+    double* to = (k == m - 1 && dst) ? dst : tp->pp[k & 1];
+    if ((rc = bs3_fixed(s, src, to, tp->t[a + k], tp->dt[a + k]))) return rc;
+    ++tp->recomputed;
+    src = to;
+  }
+  if (out) *out = src;
+  return SGNX_OK;
+}
+
+int reverse_step(sgnx_solver* s, AdjTape* tp, const double* u, long m, const double* gb) {
+  if (gb && tp->row[m] >= 0 && tp->row[m] < s->grows)
+    HIPCHK(s, launch_gauge_bar(gb + (size_t)tp->row[m] * s->nG, s->d_gnodes, s->nG, tp->lam,
+                               tp->thb, s->stream));
+  return step_adjoint(s, tp, u, tp->t[m], tp->dt[m], tp->thb);
+}
+
+// reverse the recorded steps a .. a+L-1 from u (holding u_a) with c free
+// slots; lam holds the cotangent of u_{a+L} and leaves with that of u_a
+int reverse_range(sgnx_solver* s, AdjTape* tp, const double* u, long a, long L, long c,
+                  const double* gb) {
+  int rc;
+  if (L >= 2 && c >= L - 1) {
+    // every state of the range fits: one forward pass, then the reverse
+    std::vector<double*> st(L - 1);
+    const double* prev = u;
+    for (long k = 0; k < L - 1; ++k) {
+      if ((rc = take_slot(s, tp, &st[k]))) return rc;
+      if ((rc = advance(s, tp, prev, a + k, 1, st[k], nullptr))) return rc;
+      prev = st[k];
+    }
+    for (long k = L - 1; k >= 1; --k) {
+      if ((rc = reverse_step(s, tp, st[k - 1], a + k, gb))) return rc;
+      give_slot(tp, st[k - 1]);
+    }
+    return reverse_step(s, tp, u, a, gb);
+  }
+  while (L > 0) {
+    if (L == 1) return reverse_step(s, tp, u, a, gb);
+    if (c <= 0) {
+      // no slot: recompute the last state of the range through the pair
+      const double* v = nullptr;
+      if ((rc = advance(s, tp, u, a, L - 1, nullptr, &v))) return rc;
+      if ((rc = reverse_step(s, tp, v, a + L - 1, gb))) return rc;
+      --L;
+      continue;
+    }
+    const long m = best_split(L, c);
+    double* slot = nullptr;
+    if ((rc = take_slot(s, tp, &slot))) return rc;
+    if ((rc = advance(s, tp, u, a, m, slot, nullptr))) return rc;
+    if ((rc = reverse_range(s, tp, slot, a + m, L - m, c - 1, gb))) return rc;
+    give_slot(tp, slot);
+    L = m;
+  }
+  return SGNX_OK;
+}
+
 AdjTape* tape_of(sgnx_solver* s) {
   if (!s->tape) s->tape = new (std::nothrow) AdjTape();
   return s->tape;
@@ -213,7 +338,7 @@ void tape_free(sgnx_solver* s) {
   release_snapshots(tp);
   for (double* p : tp->spare) (void)hipFree(p);
   for (double* p : tp->seg) (void)hipFree(p);
-  for (double* p : {tp->lam, tp->y3b, tp->y2b, tp->coef, tp->thb, tp->gb})
+  for (double* p : {tp->lam, tp->y3b, tp->y2b, tp->coef, tp->thb, tp->gb, tp->pp[0], tp->pp[1]})
     if (p) (void)hipFree(p);
   delete tp;
   s->tape = nullptr;
@@ -292,6 +417,22 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots) {
   tp->dt.clear();
   tp->row.clear();
   tp->gnodes = s->gnodes;
+  return SGNX_OK;
+}
+
+int sgnx_tape_set_slots(sgnx_solver* s, long slots) {
+  if (!s) return SGNX_EINVAL;
+  if (slots < -1) return fail(s, SGNX_EINVAL, "tape: slots must be >= -1");
+  AdjTape* tp = tape_of(s);
+  if (!tp) return fail(s, SGNX_ENOMEM, "host allocation failed");
+  tp->slots = slots;
+  return SGNX_OK;
+}
+
+int sgnx_adjoint_stats(const sgnx_solver* s, long* recomputed_steps, long* peak_slots) {
+  if (!s || !recomputed_steps || !peak_slots) return SGNX_EINVAL;
+  *recomputed_steps = s->tape ? s->tape->recomputed : 0;
+  *peak_slots = s->tape ? s->tape->peakSlots : 0;
   return SGNX_OK;
 }
 
@@ -376,29 +517,20 @@ int sgnx_adjoint(sgnx_solver* s, const double* Ubar_final, const double* gauge_b
       gb = gauge_bar;
     }
   }
-  // segment buffers: states K-1 steps past a snapshot
+  // snapshot segments, last first; inside a segment the states are recomputed
+  // with at most `slots` held at once (binomial schedule when that is fewer
+  // than the segment needs)
   const long K = tp->K;
-  while ((long)tp->seg.size() < std::min(K, std::max(N, 1L)) - 1) {
-    double* p = nullptr;
-    HIPCHK(s, hipMalloc((void**)&p, n * sizeof(double)));
-    tp->seg.push_back(p);
-  }
+  const long c = tp->slots < 0 ? K : tp->slots;
+  if (c < K - 1)
+    for (double** p : {&tp->pp[0], &tp->pp[1]})
+      if ((rc = alloc_state(s, p, n))) return rc;
+  tp->recomputed = 0;
+  tp->peakSlots = 0;
+  tp->inUse = 0;
   for (long j = (N + K - 1) / K - 1; j >= 0; --j) {
     const long n0 = j * K, n1 = std::min(n0 + K, N);
-    // states of the segment: u_{n0} (snapshot), u_{n0+1..n1-1} recomputed
-    std::vector<const double*> us(n1 - n0);
-    us[0] = tp->snap[j];
-    for (long m = n0 + 1; m < n1; ++m) {
-      double* dst = tp->seg[m - n0 - 1];
-      if ((rc = bs3_fixed(s, us[m - n0 - 1], dst, tp->t[m - 1], tp->dt[m - 1]))) return rc;
-      us[m - n0] = dst;
-    }
-    for (long m = n1 - 1; m >= n0; --m) {
-      if (gb && tp->row[m] >= 0 && tp->row[m] < s->grows)
-        HIPCHK(s, launch_gauge_bar(gb + (size_t)tp->row[m] * s->nG, s->d_gnodes, s->nG, tp->lam,
-                                   tp->thb, s->stream));
-      if ((rc = step_adjoint(s, tp, us[m - n0], tp->t[m], tp->dt[m], tp->thb))) return rc;
-    }
+    if ((rc = reverse_range(s, tp, tp->snap[j], n0, n1 - n0, c, gb))) return rc;
   }
   const hipMemcpyKind outk = mem == SGNX_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
   HIPCHK(s, hipMemcpyAsync(U0bar, tp->lam, n * sizeof(double), outk, s->stream));

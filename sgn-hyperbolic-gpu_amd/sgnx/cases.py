@@ -16,6 +16,9 @@ Workloads
   shoal      1024x512 on 96x48 m: 1:20 slope 0.45 -> 0.10 m plus a seeded
              elliptic Gaussian shoal, 1 s / 0.01 m incident wave on the left
              (slow-branch linear eigenmode), walls elsewhere, fluid at rest
+  inversion  256x128 on 64x32 m, 0.5 m minus six seeded mounds, walls, eight
+             seeded surface humps at rest, 20 seeded gauges (the adjoint's
+             gauge-inversion workload)
 """
 from __future__ import annotations
 
@@ -269,6 +272,48 @@ def sweep_member(seed, Nx, P=4, period=1.0, amp=0.01):
     U0 = prim_to_state(Hn.copy(), z, z, z, z)
     return Case(f"sweep{seed}", P, Nx, Ny, Lx, Ly, par, depth, bc, U0, inflow=(table, omega),
                 meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "seed": seed, "shoal": sp})
+
+
+def inversion(seed=0, Nx=256, Ny=128, P=4, n_gauges=20):
+    """The gauge-inversion workload: 256x128 elements on 64x32 m, still depth
+    H = 0.5 - (six seeded Gaussian mounds of 0.05-0.2 m), walls on every side,
+    the fluid at rest under a free surface of eight seeded Gaussian humps
+    (0.005-0.03 m high, 1-4 m wide), and n_gauges seeded gauge nodes
+    (meta["gauges"]). The depth is a nodal field: the face depths follow from
+    it as SGN2D.setDepth forms them (the mean of the two edge values)."""
+    Lx, Ly = 64.0, 32.0
+    h0 = 0.5
+    par = Params(g=G, c=ALPHA * math.sqrt(G * h0), gamma=GAMMA)
+    rng = np.random.default_rng(seed)
+    mounds = [(rng.uniform(0.1, 0.9) * Lx, rng.uniform(0.1, 0.9) * Ly, rng.uniform(0.05, 0.2),
+               rng.uniform(2.0, 6.0)) for _ in range(6)]
+    humps = [(rng.uniform(0.15, 0.85) * Lx, rng.uniform(0.15, 0.85) * Ly,
+              rng.uniform(0.005, 0.03), rng.uniform(1.0, 4.0)) for _ in range(8)]
+
+    def bumps(x, y, items):
+        out = np.zeros_like(x)
+        for xc, yc, a, w in items:
+            out = out + a * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / (w * w))
+        return out
+
+    x, y = node_coords(P, Nx, Ny, Lx, Ly)
+    theta = h0 - bumps(x, y, mounds)
+    eta0 = bumps(x, y, humps)
+    T = theta.reshape(Ny, Nx, P + 1, P + 1)
+    fx = np.empty((Ny, Nx + 1, P + 1))
+    fx[:, 1:Nx] = 0.5 * (T[:, :-1, :, P] + T[:, 1:, :, 0])
+    fx[:, 0], fx[:, Nx] = T[:, 0, :, 0], T[:, -1, :, P]
+    fy = np.empty((Ny + 1, Nx, P + 1))
+    fy[1:Ny] = 0.5 * (T[:-1, :, P, :] + T[1:, :, 0, :])
+    fy[0], fy[Ny] = T[0, :, 0, :], T[-1, :, P, :]
+    depth = {"Hnod": np.ascontiguousarray(theta), "HfaceX": fx.ravel(), "HfaceY": fy.ravel()}
+    z = np.zeros_like(theta)
+    U0 = prim_to_state(theta + eta0, z, z, z, z)
+    nd = theta.size
+    gauges = np.sort(rng.choice(nd, size=n_gauges, replace=False)).astype(np.int64)
+    return Case("inversion", P, Nx, Ny, Lx, Ly, par, depth, ("wall",) * 4, U0,
+                meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "steps": 2000, "seed": seed,
+                      "theta": theta, "eta0": eta0, "gauges": gauges, "checkpoints": 40})
 
 
 def sweep_sizes(n=64, seed=0, nx_min=128, nx_max=768):

@@ -360,6 +360,17 @@ class SGN2D:
         integrateAdaptive calls (at most max_snapshots states held)."""
         L.check(self.lib.sgnx_tape_begin(self.h, int(max_snapshots)), self.h)
 
+    def tapeSlots(self, slots):
+        """States the reverse sweep may hold besides the snapshots (-1: all a
+        segment needs; fewer: binomial recomputation, same result)."""
+        L.check(self.lib.sgnx_tape_set_slots(self.h, int(slots)), self.h)
+
+    def adjointStats(self):
+        """(steps recomputed, slots held at once) of the last adjoint()."""
+        a, b = C.c_long(), C.c_long()
+        L.check(self.lib.sgnx_adjoint_stats(self.h, C.byref(a), C.byref(b)), self.h)
+        return a.value, b.value
+
     def tapeEnd(self):
         L.check(self.lib.sgnx_tape_end(self.h), self.h)
 

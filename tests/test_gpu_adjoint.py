@@ -209,27 +209,120 @@ def test_adjoint_of_adaptive_run_matches_finite_differences(ci):
     s.close()
 
 
-def test_snapshot_thinning_gives_the_same_adjoint():
-    """A budget of 2 snapshots (stride grows to 8 over 13 steps, segments
-    recomputed) and one of 1 give the store-all cotangents bit for bit."""
+def _recompute_cost(L, c):
+    """Forward steps an optimal schedule needs to reverse L steps from a held
+    state with c free slots (dynamic programming over the first checkpoint)."""
+    from functools import lru_cache
+
+    @lru_cache(None)
+    def F(L, c):
+        if L <= 1:
+            return 0
+        if c <= 0:
+            return L * (L - 1) // 2
+        return min(m + F(L - m, c - 1) + F(m, c) for m in range(1, L))
+    return F(L, c)
+
+
+@pytest.mark.parametrize("budget,slots", [(2, -1), (1, -1), (2, 0), (2, 1), (3, 2), (1, 3)])
+def test_checkpoint_budgets_give_the_same_adjoint(budget, slots):
+    """Snapshot budgets below the step count (stride grows, segments
+    recomputed) and slot budgets below a segment's length (binomial schedule)
+    give the store-all cotangents bit for bit, recomputing exactly as many
+    steps as an optimal schedule needs."""
     c = _cases()[2]
-    rng = np.random.default_rng(7)
     theta = c.depth["Hnod"].copy()
     nodes = np.array([5, 77, 300], dtype=np.int64)
     res = []
-    for budget in (1 << 20, 2, 1):
-        s, UN, G = _taped_run(c, theta, c.U0, nodes, 13, budget)
+    for bud, sl in ((1 << 20, -1), (budget, slots)):
+        s, UN, G = _taped_run(c, theta, c.U0, nodes, 13, bud)
+        s.tapeSlots(sl)
         steps, stride, held = s.tapeInfo()
-        assert steps == 13 and held <= budget
-        if budget < 13:
-            assert stride > 1
+        assert steps == 13 and held <= bud
         r = np.random.default_rng(9)
         res.append(s.adjoint(r.normal(size=UN.size), r.normal(size=G.shape)))
+        recomputed, peak = s.adjointStats()
+        if sl >= 0:
+            assert peak <= sl
+        segs = [min(stride, 13 - j * stride) for j in range((13 + stride - 1) // stride)]
+        want = sum(_recompute_cost(L, L - 1 if sl < 0 else sl) for L in segs)
+        assert recomputed == want, (bud, sl, stride, recomputed, want)
         s.close()
-    for U0b, thb in res[1:]:
-        assert_bitwise(U0b, res[0][0], "U0bar under a snapshot budget")
-        assert_bitwise(thb, res[0][1], "thetaBar under a snapshot budget")
-    del rng
+    assert_bitwise(res[1][0], res[0][0], "U0bar under a checkpoint budget")
+    assert_bitwise(res[1][1], res[0][1], "thetaBar under a checkpoint budget")
+
+
+def test_inversion_gradient_32_directions():
+    """The gauge-inversion workload (smaller grid and run): the gradient of
+    the gauge misfit 0.5 |G(eta0, theta) - G_obs|^2 with respect to the initial
+    surface and the still depth, through 150 adaptive steps under 8 snapshots
+    and 8 slots, against central differences of the plain solver's replay of
+    the taped steps in 32 seeded directions, and in one of them against the
+    oracle's replay."""
+    import sgnx
+    from sgnx import cases
+    c = cases.inversion(seed=4, Nx=64, Ny=32)
+    theta, eta0, nodes = c.meta["theta"], c.meta["eta0"], c.meta["gauges"]
+    nd = c.nd
+    rest = c.U0[nd:]
+    steps = 150
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.setDepth(theta)
+    s.upload(c.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=steps)
+    # observations: the same run from a perturbed surface
+    s.upload(np.concatenate([theta + 1.1 * eta0, rest]))
+    s.setGauges(nodes, steps)
+    s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    G_obs = s.gaugeLog()[1]
+    s.upload(c.U0)
+    s.setGauges(nodes, steps)
+    s.tapeBegin(8)
+    st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    s.tapeEnd()
+    s.tapeSlots(8)
+    assert st.accepted == steps
+    G = s.gaugeLog()[1]
+    U0b, thb = s.adjoint(None, G - G_obs)
+    assert s.tapeInfo()[2] <= 8 and s.adjointStats()[1] <= 8
+    g_eta = U0b[:nd]                      # u0 = (theta + eta0, 0, ...)
+    g_theta = thb + U0b[:nd]
+
+    def misfit(e, th):
+        s.setDepth(th)
+        s.upload(np.concatenate([th + e, rest]))
+        s.setGauges(nodes, steps)
+        s.tapeReplay()
+        return 0.5 * np.sum((s.gaugeLog()[1] - G_obs) ** 2)
+
+    rng = np.random.default_rng(32)
+    eps = 1e-6
+    worst = 0.0
+    for k in range(32):
+        de = rng.normal(size=nd) * 0.01
+        dth = rng.normal(size=nd) * 0.01
+        fd = (misfit(eta0 + eps * de, theta + eps * dth) -
+              misfit(eta0 - eps * de, theta - eps * dth)) / (2 * eps)
+        ad = g_eta @ de + g_theta @ dth
+        worst = max(worst, _rel(ad, fd))
+    assert worst < TOL, worst
+    # one direction against the oracle's replay of the same steps
+    tt, dts = s.tapeSteps()
+
+    def misfit_oracle(e, th):
+        o = _oracle(c, th)
+        U = np.concatenate([th + e, rest])
+        rows = []
+        for t, dt in zip(tt, dts):
+            U = o.bs3_step(U, t, dt)
+            rows.append(U[nodes] - th[nodes])
+        o.close()
+        return 0.5 * np.sum((np.array(rows) - G_obs) ** 2)
+    fdo = (misfit_oracle(eta0 + eps * de, theta + eps * dth) -
+           misfit_oracle(eta0 - eps * de, theta - eps * dth)) / (2 * eps)
+    assert _rel(g_eta @ de + g_theta @ dth, fdo) < TOL
+    s.close()
 
 
 def test_autograd_function_matches_finite_differences():
