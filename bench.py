@@ -303,6 +303,101 @@ def adjoint_leg(args):
     return out
 
 
+def inversion_leg(args):
+    """The gauge-inversion workload (cases.inversion: 256x128, walls, surface
+    humps at rest, 20 gauges): the gradient of the gauge misfit
+    0.5 |G - G_obs|^2 with respect to the initial surface eta0 and the still
+    depth theta over --inversion-steps adaptive steps (atol 1e-10, rtol 1e-8),
+    under a budget of 40
+    stored states (20 snapshots + 20 recomputation slots). G_obs is the same
+    run from 1.1 eta0. Reports its cost against the plain run, and the largest
+    relative difference to central differences of the plain solver's replay
+    of the taped steps over --inversion-dirs seeded smooth directions (held
+    step sizes leave rough perturbations' grid modes unstable)."""
+    import numpy as np
+    import sgnx
+    from sgnx import cases
+    c = cases.inversion(seed=args.seed)
+    theta, eta0, nodes = c.meta["theta"], c.meta["eta0"], c.meta["gauges"]
+    nd, steps = c.nd, args.inversion_steps
+    rest = c.U0[nd:]
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.setDepth(theta)
+    s.upload(c.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = sgnx.RkOpts(atol=c.meta["atol"], rtol=c.meta["rtol"], max_accepted=steps)
+    s.upload(np.concatenate([theta + 1.1 * eta0, rest]))
+    s.setGauges(nodes, steps)
+    s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    G_obs = s.gaugeLog()[1]
+    s.upload(c.U0)
+    s.setGauges(nodes, steps)
+    t0 = time.perf_counter()
+    st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    t_fwd = time.perf_counter() - t0
+    s.upload(c.U0)
+    s.setGauges(nodes, steps)
+    s.tapeBegin(20)
+    t0 = time.perf_counter()
+    st1 = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    t_tape = time.perf_counter() - t0
+    s.tapeEnd()
+    s.tapeSlots(20)
+    G = s.gaugeLog()[1]
+    t0 = time.perf_counter()
+    U0b, thb = s.adjoint(None, G - G_obs)
+    t_adj = time.perf_counter() - t0
+    recomputed, peak = s.adjointStats()
+    _, stride, held = s.tapeInfo()
+    g_eta, g_theta = U0b[:nd], thb + U0b[:nd]
+
+    def misfit(e, th):
+        s.setDepth(th)
+        s.upload(np.concatenate([th + e, rest]))
+        s.setGauges(nodes, steps)
+        s.tapeReplay()
+        return 0.5 * np.sum((s.gaugeLog()[1] - G_obs) ** 2)
+
+    rng = np.random.default_rng(args.seed + 32)
+    eps, errs, rels = 3e-3, [], []
+    t0 = time.perf_counter()
+    for _ in range(args.inversion_dirs):
+        de = cases.smooth_direction(c, rng, near=nodes)
+        dth = cases.smooth_direction(c, rng, near=nodes)
+        # fourth-order central difference: truncation O(eps^4)
+        f = {k: misfit(eta0 + k * eps * de, theta + k * eps * dth) for k in (-2, -1, 1, 2)}
+        fd = (8.0 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12.0 * eps)
+        pe, pt = g_eta @ de, g_theta @ dth
+        errs.append(abs(pe + pt - fd) / (abs(pe) + abs(pt)))
+        rels.append(abs(pe + pt - fd) / max(abs(pe + pt), abs(fd), 1e-300))
+    t_fd = time.perf_counter() - t0
+    s.close()
+    out = {"workload": (f"inversion {c.Nx}x{c.Ny} on 64x32 m, walls, 8 surface humps at rest, "
+                        f"{nodes.size} gauges, {steps} accepted steps"),
+           "accepted": st.accepted, "rejected": st.rejected,
+           "same_step_counts": st.as_dict() == st1.as_dict(),
+           "stored_states": {"snapshots": held, "stride": stride, "slots_peak": peak,
+                             "budget": 40},
+           "recomputed_steps": recomputed,
+           "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "adjoint_ms": 1e3 * t_adj,
+           "adjoint_over_forward": t_adj / t_fwd,
+           "gradient_over_forward": (t_tape + t_adj) / t_fwd,
+           "fd_directions": len(errs), "fd_max_rel_err": max(errs) if errs else None,
+           "fd_median_rel_err": float(np.median(errs)) if errs else None,
+           "fd_error_norm": "|adjoint - fd| / (|eta0 part| + |theta part|) of the directional derivative",
+           "fd_max_plain_rel_err": max(rels) if rels else None,
+           "fd_median_plain_rel_err": float(np.median(rels)) if rels else None,
+           "fd_reference": ("fourth-order central differences (eps 3e-3) of the plain solver's "
+                            "replay of the taped steps, along seeded smooth directions (4 "
+                            "Gaussians each for eta0 and theta, centred near gauges)")}
+    log(f"[bench] inversion: {st.accepted} steps, forward {1e3 * t_fwd:.0f} ms, taped "
+        f"{1e3 * t_tape:.0f} ms, reverse {1e3 * t_adj:.0f} ms ({held} snapshots every {stride}, "
+        f"{peak} slots, {recomputed} steps recomputed) -> gradient/forward "
+        f"{(t_tape + t_adj) / t_fwd:.2f}; FD over {len(errs)} directions: max rel err "
+        f"{max(errs) if errs else float('nan'):.2e} ({t_fd:.1f}s)")
+    return out
+
+
 def cpu_baseline(case, gpu_solver, dt0, args):
     """The oracle on one core: FSAL head + 2 accepted steps of the same workload."""
     import numpy as np
@@ -354,6 +449,10 @@ def main():
                     help="accepted steps per member in the batched sweep leg")
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="accepted steps of the gradient leg (0 = skip)")
+    ap.add_argument("--inversion-steps", type=int, default=2000,
+                    help="accepted steps of the gauge-inversion gradient leg (0 = skip)")
+    ap.add_argument("--inversion-dirs", type=int, default=32,
+                    help="finite-difference directions checked on the inversion gradient")
     ap.add_argument("--profile-rhs", type=int, default=0)
     ap.add_argument("--profile-vjp", type=int, default=0,
                     help="profile mode: this many RHS vector-Jacobian products")
@@ -449,6 +548,14 @@ def main():
             adjoint = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] adjoint leg failed: {adjoint['error']}")
 
+    inversion = None
+    if args.inversion_steps > 0:
+        try:
+            inversion = inversion_leg(args)
+        except Exception as e:                    # reported, not fatal to the main line
+            inversion = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] inversion leg failed: {inversion['error']}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -493,6 +600,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "inversion": inversion,
     }
     print(json.dumps(out), flush=True)
 
