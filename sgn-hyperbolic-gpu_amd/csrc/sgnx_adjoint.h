@@ -33,6 +33,38 @@ struct VjpArgs {
 
 inline long long vjp_coef_doubles(const Geometry& g) { return 120LL * g.Nx * g.Ny; }
 
+// Batched forms (one launch over every member, tiles packed behind a
+// per-member offset table as in the batched RHS): a product per member ...
+struct VjpMember {
+  VjpArgs va;
+  Geometry g;
+  Phys ph;
+  GhostTime gt;
+};
+// ... a stage combination out = u + a k per member ...
+struct CombMember {
+  double* out;
+  const double* u;
+  double a;
+  const double* k;
+  long long n;
+};
+// ... and one gauge log row's cotangents per member (Gb null: none)
+struct GaugeMember {
+  const double* Gb;
+  const long long* nodes;
+  int n;
+  double* lam;
+  double* thetaBar;
+};
+long long vjp_tiles(const Geometry& g);       // workgroups of one product launch
+long long comb_blocks(long long n);           // workgroups of a combination of n doubles
+hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart, int nMem,
+                                long long totalTiles, hipStream_t st);
+hipError_t launch_comb_batch(const CombMember* mem, const long long* blockStart, int nMem,
+                             long long totalBlocks, hipStream_t st);
+hipError_t launch_gauge_bar_batch(const GaugeMember* gm, int nMem, hipStream_t st);
+
 hipError_t launch_rhs_vjp(const Geometry& g, const Phys& ph, const VjpArgs& va, GhostTime gt,
                           hipStream_t st);
 // HfX, HfY from the nodal still depth theta: the mean of the two edge values

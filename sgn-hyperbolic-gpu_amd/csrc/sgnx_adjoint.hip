@@ -242,14 +242,14 @@ __device__ __forceinline__ void tile_stage(const VjpArgs& va, const Geometry& g,
 
 // ---- launch 1: face-line cotangents -----------------------------------------------
 // coef[((e * 4 + f) * NP + n) * 6 + c]: c < 5 Lambda_c, c = 5 lambda
-__global__ void __launch_bounds__(kT)
-k_vjp_coef(VjpArgs va, Geometry g, Phys ph) {
+__device__ __forceinline__ void vjp_coef_tile(const VjpArgs& va, const Geometry& g,
+                                              const Phys& ph, int tile) {
   __shared__ TileLds L;
   const int t = threadIdx.x;
   const int Nx = g.Nx;
   const int strips = (Nx + kEW - 1) / kEW;
-  const int ey = blockIdx.x / strips;
-  const int ex0 = (blockIdx.x - ey * strips) * kEW;
+  const int ey = tile / strips;
+  const int ex0 = (tile - ey * strips) * kEW;
   const int nW = min(kEW, Nx - ex0);
   const int j = t / NPP;
   const long long i = ((long long)ey * Nx + ex0) * NPP + t;
@@ -268,8 +268,8 @@ k_vjp_coef(VjpArgs va, Geometry g, Phys ph) {
 }
 
 // ---- launch 2: one element row strip per block ---------------------------------
-__global__ void __launch_bounds__(kT)
-k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
+__device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry& g,
+                                              const Phys& ph, const GhostTime& gt, int tile) {
   __shared__ TileLds L;
   __shared__ double sH[kEW * NPP];
   __shared__ double sHS[kEW * 20], sLam[kEW * 20], sTF[kEW * 20];
@@ -278,8 +278,8 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
   const int Nx = g.Nx, Ny = g.Ny;
   const long long Nd = g.Nd;
   const int strips = (Nx + kEW - 1) / kEW;
-  const int ey = blockIdx.x / strips;
-  const int ex0 = (blockIdx.x - ey * strips) * kEW;
+  const int ey = tile / strips;
+  const int ex0 = (tile - ey * strips) * kEW;
   const int nW = min(kEW, Nx - ex0);
   const double* U = va.U;
   const double* sD = L.D;
@@ -507,6 +507,73 @@ k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
   }
 }
 
+__global__ void __launch_bounds__(kT)
+k_vjp_coef(VjpArgs va, Geometry g, Phys ph) {
+  vjp_coef_tile(va, g, ph, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kT)
+k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
+  vjp_elem_tile(va, g, ph, gt, blockIdx.x);
+}
+
+// ---- batches: every member's tiles packed into one launch -------------------------
+// the member owning packed tile w: tileStart[m] <= w < tileStart[m + 1]
+__device__ __forceinline__ int member_of(const long long* tileStart, int nMem, long long w) {
+  int lo = 0, hi = nMem - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tileStart[mid] <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kT)
+k_vjp_coef_batch(const VjpMember* __restrict__ mem, const long long* __restrict__ tileStart,
+                 int nMem) {
+  const long long w = blockIdx.x;
+  const int m = member_of(tileStart, nMem, w);
+  vjp_coef_tile(mem[m].va, mem[m].g, mem[m].ph, (int)(w - tileStart[m]));
+}
+
+__global__ void __launch_bounds__(kT)
+k_vjp_elem_batch(const VjpMember* __restrict__ mem, const long long* __restrict__ tileStart,
+                 int nMem) {
+  const long long w = blockIdx.x;
+  const int m = member_of(tileStart, nMem, w);
+  vjp_elem_tile(mem[m].va, mem[m].g, mem[m].ph, mem[m].gt, (int)(w - tileStart[m]));
+}
+
+// out = u + a k over every member's n doubles; blockStart over chunks of kT * 4
+constexpr int kCombChunk = kT * 4;
+__global__ void __launch_bounds__(kT)
+k_comb_batch(const CombMember* __restrict__ mem, const long long* __restrict__ blockStart,
+             int nMem) {
+  const long long w = blockIdx.x;
+  const int m = member_of(blockStart, nMem, w);
+  const CombMember c = mem[m];
+  const long long base = (w - blockStart[m]) * kCombChunk;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const long long i = base + r * kT + threadIdx.x;
+    if (i < c.n) c.out[i] = c.u[i] + c.a * c.k[i];
+  }
+}
+
+// the gauge cotangents of one log row per member (one block per member, one
+// thread, gauge order)
+__global__ void k_gauge_bar_batch(const GaugeMember* __restrict__ gm, int nMem) {
+  if (threadIdx.x != 0 || (int)blockIdx.x >= nMem) return;
+  const GaugeMember q = gm[blockIdx.x];
+  if (!q.Gb) return;
+  for (int k = 0; k < q.n; ++k) {
+    const long long i = q.nodes[k];
+    q.lam[i] += q.Gb[k];
+    if (q.thetaBar) q.thetaBar[i] -= q.Gb[k];
+  }
+}
+
 // ---- still depth from theta -------------------------------------------------------
 __global__ void __launch_bounds__(kT)
 k_face_depth(const double* __restrict__ th, int Nx, int Ny, int perX, int perY,
@@ -559,6 +626,36 @@ hipError_t launch_rhs_vjp(const Geometry& g, const Phys& ph, const VjpArgs& va, 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_vjp_elem, dim3((unsigned)nb), dim3(kT), 0, st, va, g, ph, gt);
+  return hipGetLastError();
+}
+
+long long vjp_tiles(const Geometry& g) { return (long long)g.Ny * ((g.Nx + kEW - 1) / kEW); }
+
+long long comb_blocks(long long n) { return (n + kCombChunk - 1) / kCombChunk; }
+
+hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart, int nMem,
+                                long long totalTiles, hipStream_t st) {
+  if (totalTiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vjp_coef_batch, dim3((unsigned)totalTiles), dim3(kT), 0, st, mem,
+                     tileStart, nMem);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_vjp_elem_batch, dim3((unsigned)totalTiles), dim3(kT), 0, st, mem,
+                     tileStart, nMem);
+  return hipGetLastError();
+}
+
+hipError_t launch_comb_batch(const CombMember* mem, const long long* blockStart, int nMem,
+                             long long totalBlocks, hipStream_t st) {
+  if (totalBlocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_comb_batch, dim3((unsigned)totalBlocks), dim3(kT), 0, st, mem, blockStart,
+                     nMem);
+  return hipGetLastError();
+}
+
+hipError_t launch_gauge_bar_batch(const GaugeMember* gm, int nMem, hipStream_t st) {
+  if (nMem <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gauge_bar_batch, dim3((unsigned)nMem), dim3(64), 0, st, gm, nMem);
   return hipGetLastError();
 }
 

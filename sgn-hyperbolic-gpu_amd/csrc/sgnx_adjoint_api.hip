@@ -30,28 +30,7 @@
 #include <string>
 #include <vector>
 
-#include "sgnx_adjoint.h"
-#include "sgnx_handle.h"
-
-struct AdjTape {
-  bool recording = false;
-  long maxSnap = 0;
-  long K = 1;                      // snapshot stride in steps
-  std::vector<double*> snap;       // snap[j] = input state of step j*K
-  std::vector<double*> spare;      // allocated, unused snapshot buffers
-  std::vector<double> t, dt;       // accepted steps
-  std::vector<long> row;           // gauge log row written after step n, or -1
-  std::vector<long long> gnodes;   // gauge nodes while recording
-  // reverse-sweep buffers
-  double *lam = nullptr, *y3b = nullptr, *y2b = nullptr, *coef = nullptr, *thb = nullptr;
-  double* gb = nullptr;            // gauge cotangents staged from the host
-  size_t gbCap = 0;
-  long slots = -1;                 // states the sweep may hold besides the snapshots (-1: any)
-  std::vector<double*> seg;        // recomputation slots (allocated)
-  std::vector<double*> freeSeg;    // of those, the unused ones
-  double* pp[2] = {nullptr, nullptr};   // ping-pong pair for slot-free recomputation
-  long recomputed = 0, peakSlots = 0, inUse = 0;   // of the last sweep
-};
+#include "sgnx_tape.h"
 
 namespace {
 
@@ -295,6 +274,62 @@ void release_snapshots(AdjTape* tp) {
 
 }  // namespace
 
+int adjoint_begin(sgnx_solver* s, bool storeAll, AdjTape** out) {
+  AdjTape* tp = s->tape;
+  if (!tp || tp->recording) return fail(s, SGNX_EINVAL, "adjoint: no finished tape");
+  int rc;
+  if ((rc = check_adjoint_support(s))) return rc;
+  const long N = (long)tp->t.size();
+  if ((long)tp->snap.size() < (N + tp->K - 1) / tp->K)
+    return fail(s, SGNX_EINVAL, "adjoint: the tape lacks snapshots");
+  if (storeAll && N > 0 && tp->K != 1)
+    return fail(s, SGNX_EINVAL,
+                "adjoint: a batched sweep needs every step's snapshot (max_snapshots >= steps)");
+  if ((rc = ensure_vjp_buffers(s, tp))) return rc;
+  for (double** p : {&tp->lam, &tp->y3b, &tp->y2b})
+    if ((rc = alloc_state(s, p, s->n()))) return rc;
+  *out = tp;
+  return SGNX_OK;
+}
+
+int adjoint_inputs(sgnx_solver* s, AdjTape* tp, const double* Ubar_final, const double* gauge_bar,
+                   int mem, hipStream_t st, const double** gb) {
+  const size_t n = s->n(), Nd = (size_t)s->g.Nd;
+  if (gauge_bar && tp->gnodes != s->gnodes)
+    return fail(s, SGNX_EINVAL, "adjoint: the gauge nodes differ from those of the recording");
+  const hipMemcpyKind in = mem == SGNX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  if (Ubar_final) HIPCHK(s, hipMemcpyAsync(tp->lam, Ubar_final, n * sizeof(double), in, st));
+  else HIPCHK(s, hipMemsetAsync(tp->lam, 0, n * sizeof(double), st));
+  HIPCHK(s, hipMemsetAsync(tp->thb, 0, Nd * sizeof(double), st));
+  *gb = nullptr;
+  const size_t gdoubles = (size_t)s->grows * s->nG;
+  if (gauge_bar && gdoubles > 0) {
+    if (mem == SGNX_MEM_HOST) {
+      if (tp->gbCap < gdoubles) {
+        if (tp->gb) (void)hipFree(tp->gb);
+        tp->gb = nullptr;
+        tp->gbCap = 0;
+        HIPCHK(s, hipMalloc((void**)&tp->gb, gdoubles * sizeof(double)));
+        tp->gbCap = gdoubles;
+      }
+      HIPCHK(s, hipMemcpyAsync(tp->gb, gauge_bar, gdoubles * sizeof(double), in, st));
+      *gb = tp->gb;
+    } else {
+      *gb = gauge_bar;
+    }
+  }
+  return SGNX_OK;
+}
+
+int adjoint_outputs(sgnx_solver* s, AdjTape* tp, double* U0bar, double* theta_bar, int mem,
+                    hipStream_t st) {
+  const hipMemcpyKind outk = mem == SGNX_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  HIPCHK(s, hipMemcpyAsync(U0bar, tp->lam, s->n() * sizeof(double), outk, st));
+  if (theta_bar)
+    HIPCHK(s, hipMemcpyAsync(theta_bar, tp->thb, (size_t)s->g.Nd * sizeof(double), outk, st));
+  return SGNX_OK;
+}
+
 bool tape_recording(const sgnx_solver* s) { return s->tape && s->tape->recording; }
 
 int tape_step_begin(sgnx_solver* s) {
@@ -483,40 +518,13 @@ int sgnx_adjoint(sgnx_solver* s, const double* Ubar_final, const double* gauge_b
   if (!s || !U0bar) return SGNX_EINVAL;
   if (mem != SGNX_MEM_HOST && mem != SGNX_MEM_DEVICE)
     return fail(s, SGNX_EINVAL, "adjoint: bad memory kind");
-  AdjTape* tp = s->tape;
-  if (!tp || tp->recording) return fail(s, SGNX_EINVAL, "adjoint: no finished tape");
+  AdjTape* tp = nullptr;
   int rc;
-  if ((rc = check_adjoint_support(s))) return rc;
-  const long N = (long)tp->t.size();
-  if ((long)tp->snap.size() < (N + tp->K - 1) / tp->K)
-    return fail(s, SGNX_EINVAL, "adjoint: the tape lacks snapshots");
-  if (gauge_bar && tp->gnodes != s->gnodes)
-    return fail(s, SGNX_EINVAL, "adjoint: the gauge nodes differ from those of the recording");
-  const size_t n = s->n(), Nd = (size_t)s->g.Nd;
-  if ((rc = ensure_vjp_buffers(s, tp))) return rc;
-  for (double** p : {&tp->lam, &tp->y3b, &tp->y2b})
-    if ((rc = alloc_state(s, p, n))) return rc;
-  const hipMemcpyKind in = mem == SGNX_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-  if (Ubar_final) HIPCHK(s, hipMemcpyAsync(tp->lam, Ubar_final, n * sizeof(double), in, s->stream));
-  else HIPCHK(s, hipMemsetAsync(tp->lam, 0, n * sizeof(double), s->stream));
-  HIPCHK(s, hipMemsetAsync(tp->thb, 0, Nd * sizeof(double), s->stream));
+  if ((rc = adjoint_begin(s, false, &tp))) return rc;
   const double* gb = nullptr;
-  const size_t gdoubles = (size_t)s->grows * s->nG;
-  if (gauge_bar && gdoubles > 0) {
-    if (mem == SGNX_MEM_HOST) {
-      if (tp->gbCap < gdoubles) {
-        if (tp->gb) (void)hipFree(tp->gb);
-        tp->gb = nullptr;
-        tp->gbCap = 0;
-        HIPCHK(s, hipMalloc((void**)&tp->gb, gdoubles * sizeof(double)));
-        tp->gbCap = gdoubles;
-      }
-      HIPCHK(s, hipMemcpyAsync(tp->gb, gauge_bar, gdoubles * sizeof(double), in, s->stream));
-      gb = tp->gb;
-    } else {
-      gb = gauge_bar;
-    }
-  }
+  if ((rc = adjoint_inputs(s, tp, Ubar_final, gauge_bar, mem, s->stream, &gb))) return rc;
+  const long N = (long)tp->t.size();
+  const size_t n = s->n();
   // snapshot segments, last first; inside a segment the states are recomputed
   // with at most `slots` held at once (binomial schedule when that is fewer
   // than the segment needs)
@@ -532,10 +540,7 @@ int sgnx_adjoint(sgnx_solver* s, const double* Ubar_final, const double* gauge_b
     const long n0 = j * K, n1 = std::min(n0 + K, N);
     if ((rc = reverse_range(s, tp, tp->snap[j], n0, n1 - n0, c, gb))) return rc;
   }
-  const hipMemcpyKind outk = mem == SGNX_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  HIPCHK(s, hipMemcpyAsync(U0bar, tp->lam, n * sizeof(double), outk, s->stream));
-  if (theta_bar)
-    HIPCHK(s, hipMemcpyAsync(theta_bar, tp->thb, Nd * sizeof(double), outk, s->stream));
+  if ((rc = adjoint_outputs(s, tp, U0bar, theta_bar, mem, s->stream))) return rc;
   HIPCHK(s, hipStreamSynchronize(s->stream));
   return SGNX_OK;
 }

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "sgnx_handle.h"
+#include "sgnx_tape.h"
 
 struct sgnx_batch {
   std::vector<sgnx_solver*> m;
@@ -43,6 +44,10 @@ struct sgnx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int> graphSet;
+  // argument arena of the batched reverse sweep (pinned host, device)
+  char* h_adj = nullptr;
+  char* d_adj = nullptr;
+  size_t adjCap = 0;
   std::string err;
 };
 
@@ -189,7 +194,8 @@ int sgnx_batch_destroy(sgnx_batch* b) {
   if (b->h_mem) (void)hipHostFree(b->h_mem);
   if (b->h_ts) (void)hipHostFree(b->h_ts);
   if (b->h_out) (void)hipHostFree(b->h_out);
-  for (void* p : {(void*)b->d_mem, (void*)b->d_ts, (void*)b->d_out})
+  if (b->h_adj) (void)hipHostFree(b->h_adj);
+  for (void* p : {(void*)b->d_mem, (void*)b->d_ts, (void*)b->d_out, (void*)b->d_adj})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -465,6 +471,162 @@ int sgnx_batch_autotune(sgnx_batch* b, int reps, int* rows_out, int* waves3_out)
   for (sgnx_solver* s : b->m) s->g.waves3 = b->waves3;
   if (rows_out) *rows_out = bestRows;
   if (waves3_out) *waves3_out = b->waves3;
+  return SGNX_OK;
+}
+
+// The reverse sweeps of every member's tape in lockstep, aligned at the last
+// step: iteration k reverses step N_m - 1 - k of each member that has it, with
+// one launch per kind over all of them (the gauge cotangents, the two plain
+// RHS recomputations, the two stage combinations, the three products), each
+// member on its own buffers. The arithmetic is that of sgnx_adjoint on each
+// member alone, so the cotangents are bit for bit the same. Every member's
+// tape must hold every step's snapshot.
+int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
+                       const double* const* gauge_bar, double* const* U0bar,
+                       double* const* theta_bar, int mem) {
+  if (!b || !U0bar) return SGNX_EINVAL;
+  if (mem != SGNX_MEM_HOST && mem != SGNX_MEM_DEVICE)
+    return bfail(b, SGNX_EINVAL, "batch adjoint: bad memory kind");
+  const int n = b->n;
+  BCHK(b, hipSetDevice(b->device));
+  std::vector<AdjTape*> tp(n, nullptr);
+  std::vector<const double*> gb(n, nullptr);
+  long Nmax = 0;
+  for (int i = 0; i < n; ++i) {
+    sgnx_solver* s = b->m[i];
+    if (!U0bar[i]) return bfail(b, SGNX_EINVAL, "batch adjoint: U0bar missing for a member");
+    BCHK(b, hipStreamSynchronize(s->stream));
+    int rc = adjoint_begin(s, true, &tp[i]);
+    if (!rc)
+      rc = adjoint_inputs(s, tp[i], Ubar_final ? Ubar_final[i] : nullptr,
+                          gauge_bar ? gauge_bar[i] : nullptr, mem, b->stream, &gb[i]);
+    if (rc) return bfail(b, rc, "batch adjoint, member " + std::to_string(i) + ": " + s->err);
+    Nmax = std::max(Nmax, (long)tp[i]->t.size());
+  }
+  // argument arena: gauges, two RHS tables + offsets, two combinations +
+  // offsets, three products + offsets
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t nn = (size_t)n;
+  const size_t oG = 0, oR1 = up(oG + nn * sizeof(GaugeMember));
+  const size_t oR2 = up(oR1 + nn * sizeof(BatchMember)), oRT = up(oR2 + nn * sizeof(BatchMember));
+  const size_t oC2 = up(oRT + (nn + 1) * sizeof(long long));
+  const size_t oC3 = up(oC2 + nn * sizeof(CombMember)), oCT = up(oC3 + nn * sizeof(CombMember));
+  const size_t oV3 = up(oCT + (nn + 1) * sizeof(long long));
+  const size_t oV2 = up(oV3 + nn * sizeof(VjpMember)), oV1 = up(oV2 + nn * sizeof(VjpMember));
+  const size_t oVT = up(oV1 + nn * sizeof(VjpMember));
+  const size_t total = up(oVT + (nn + 1) * sizeof(long long));
+  if (b->adjCap < total) {
+    if (b->h_adj) (void)hipHostFree(b->h_adj);
+    if (b->d_adj) (void)hipFree(b->d_adj);
+    b->h_adj = nullptr;
+    b->d_adj = nullptr;
+    b->adjCap = 0;
+    BCHK(b, hipHostMalloc((void**)&b->h_adj, total, hipHostMallocDefault));
+    BCHK(b, hipMalloc((void**)&b->d_adj, total));
+    b->adjCap = total;
+  }
+  auto* hG = reinterpret_cast<GaugeMember*>(b->h_adj + oG);
+  auto* hR1 = reinterpret_cast<BatchMember*>(b->h_adj + oR1);
+  auto* hR2 = reinterpret_cast<BatchMember*>(b->h_adj + oR2);
+  auto* hRT = reinterpret_cast<long long*>(b->h_adj + oRT);
+  auto* hC2 = reinterpret_cast<CombMember*>(b->h_adj + oC2);
+  auto* hC3 = reinterpret_cast<CombMember*>(b->h_adj + oC3);
+  auto* hCT = reinterpret_cast<long long*>(b->h_adj + oCT);
+  auto* hV3 = reinterpret_cast<VjpMember*>(b->h_adj + oV3);
+  auto* hV2 = reinterpret_cast<VjpMember*>(b->h_adj + oV2);
+  auto* hV1 = reinterpret_cast<VjpMember*>(b->h_adj + oV1);
+  auto* hVT = reinterpret_cast<long long*>(b->h_adj + oVT);
+  auto dptr = [&](size_t off) { return b->d_adj + off; };
+  for (long k = 0; k < Nmax; ++k) {
+    int A = 0;
+    long long rAcc = 0, cAcc = 0, vAcc = 0;
+    for (int i = 0; i < n; ++i) {
+      AdjTape* T = tp[i];
+      const long m = (long)T->t.size() - 1 - k;
+      if (m < 0) continue;
+      sgnx_solver* s = b->m[i];
+      const double t = T->t[m], dt = T->dt[m];
+      const double* u = T->snap[m];
+      const long long nst = (long long)s->n();
+      GaugeMember gq{};
+      if (gb[i] && T->row[m] >= 0 && T->row[m] < s->grows) {
+        gq.Gb = gb[i] + (size_t)T->row[m] * s->nG;
+        gq.nodes = s->d_gnodes;
+        gq.n = s->nG;
+        gq.lam = T->lam;
+        gq.thetaBar = T->thb;
+      }
+      hG[A] = gq;
+      Stage s1{};
+      s1.u = u;
+      hR1[A] = member_args(s, u, s->d_k1, s1, t);
+      Stage s2{};
+      s2.u = s->d_tmp;
+      hR2[A] = member_args(s, s->d_tmp, s->d_k2, s2, t + 0.5 * dt);
+      hRT[A] = rAcc;
+      rAcc += sweep_tiles(s->g);
+      hC2[A] = CombMember{s->d_tmp, u, 0.5 * dt, s->d_k1, nst};
+      hC3[A] = CombMember{s->d_k3, u, 0.75 * dt, s->d_k2, nst};
+      hCT[A] = cAcc;
+      cAcc += comb_blocks(nst);
+      // uNew = u + dt ((2/9 k1 + 1/3 k2) + 4/9 k3)   (rk.cpp:106-108)
+      VjpArgs a3{};
+      a3.U = s->d_k3;
+      a3.A = T->lam;
+      a3.ca = dt * (4.0 / 9.0);
+      a3.out = T->y3b;
+      a3.thetaBar = T->thb;
+      a3.coef = T->coef;
+      VjpArgs a2 = a3;
+      a2.U = s->d_tmp;
+      a2.ca = dt * (1.0 / 3.0);
+      a2.B = T->y3b;
+      a2.cb = 0.75 * dt;
+      a2.out = T->y2b;
+      VjpArgs a1 = a3;
+      a1.U = u;
+      a1.ca = dt * (2.0 / 9.0);
+      a1.B = T->y2b;
+      a1.cb = 0.5 * dt;
+      a1.out = T->lam;
+      a1.acc0 = T->lam;
+      a1.acc1 = T->y3b;
+      a1.acc2 = T->y2b;
+      hV3[A] = VjpMember{a3, s->g, s->ph, ghost_time(s, t + 0.75 * dt)};
+      hV2[A] = VjpMember{a2, s->g, s->ph, ghost_time(s, t + 0.5 * dt)};
+      hV1[A] = VjpMember{a1, s->g, s->ph, ghost_time(s, t)};
+      hVT[A] = vAcc;
+      vAcc += vjp_tiles(s->g);
+      ++A;
+    }
+    hRT[A] = rAcc;
+    hCT[A] = cAcc;
+    hVT[A] = vAcc;
+    BCHK(b, hipMemcpyAsync(b->d_adj, b->h_adj, total, hipMemcpyHostToDevice, b->stream));
+    const auto* dR1 = reinterpret_cast<const BatchMember*>(dptr(oR1));
+    const auto* dR2 = reinterpret_cast<const BatchMember*>(dptr(oR2));
+    const auto* dRT = reinterpret_cast<const long long*>(dptr(oRT));
+    const auto* dCT = reinterpret_cast<const long long*>(dptr(oCT));
+    const auto* dVT = reinterpret_cast<const long long*>(dptr(oVT));
+    BCHK(b, launch_gauge_bar_batch(reinterpret_cast<const GaugeMember*>(dptr(oG)), A, b->stream));
+    BCHK(b, launch_rhs_stage_batch(dR1, dRT, A, rAcc, kPlain, b->waves3 & 1, b->stream));
+    BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC2)), dCT, A, cAcc,
+                              b->stream));
+    BCHK(b, launch_rhs_stage_batch(dR2, dRT, A, rAcc, kPlain, b->waves3 & 1, b->stream));
+    BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC3)), dCT, A, cAcc,
+                              b->stream));
+    for (size_t o : {oV3, oV2, oV1})
+      BCHK(b, launch_rhs_vjp_batch(reinterpret_cast<const VjpMember*>(dptr(o)), dVT, A, vAcc,
+                                   b->stream));
+    // the arena is refilled for the next iteration
+    BCHK(b, hipStreamSynchronize(b->stream));
+  }
+  for (int i = 0; i < n; ++i) {
+    int rc = adjoint_outputs(b->m[i], tp[i], U0bar[i], theta_bar ? theta_bar[i] : nullptr, mem,
+                             b->stream);
+    if (rc) return bfail(b, rc, "batch adjoint: " + b->m[i]->err);
+  }
+  BCHK(b, hipStreamSynchronize(b->stream));
   return SGNX_OK;
 }
 

@@ -110,6 +110,8 @@ SIGNATURES = [
     ("sgnx_batch_rhs", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _dp]),
     ("sgnx_batch_integrate", C.c_int, [C.c_void_p, _dp, _dp, _dp, C.POINTER(RkOpts),
                                        C.POINTER(RkStats), C.c_int]),
+    ("sgnx_batch_adjoint", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]),
     ("sgnx_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("sgnx_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_int)]),

@@ -88,6 +88,37 @@ class Batch:
             a2.ctypes.data_as(L._dp), oa, st, flags))
         return [st[i] for i in range(n)]
 
+    def adjoint(self, Ubar_final=None, gauge_bar=None, theta=True):
+        """Every member's SGN2D.adjoint (its tape recorded with a snapshot of
+        every step), the reverse sweeps in lockstep with one launch per kind;
+        each member's result is bit for bit its own adjoint's. Ubar_final,
+        gauge_bar: None or one array (or None) per member. Returns a list of
+        (U0bar, thetaBar) (or U0bar when theta is False)."""
+        n = len(self.members)
+        keep = []
+
+        def ptrs(arrs, sizes):
+            out = (C.c_void_p * n)()
+            for i in range(n):
+                a = None if arrs is None else arrs[i]
+                if a is not None:
+                    a = np.ascontiguousarray(a, dtype=np.float64)
+                    if sizes is not None and a.size != sizes[i]:
+                        raise ValueError(f"member {i}: {a.size} values, expected {sizes[i]}")
+                    keep.append(a)
+                    out[i] = a.ctypes.data
+            return out
+        U0b = [np.empty(m.nTotal()) for m in self.members]
+        thb = [np.empty(m.nDof()) for m in self.members] if theta else None
+        uf = ptrs(Ubar_final, [m.nTotal() for m in self.members])
+        gb = ptrs(gauge_bar, None)
+        ub = ptrs(U0b, None)
+        tb = ptrs(thb, None) if theta else None
+        self._check(self.lib.sgnx_batch_adjoint(
+            self.h, uf if Ubar_final is not None else None,
+            gb if gauge_bar is not None else None, ub, tb, L.MEM_HOST))
+        return list(zip(U0b, thb)) if theta else U0b
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.sgnx_batch_destroy(self.h)

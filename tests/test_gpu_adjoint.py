@@ -385,3 +385,44 @@ def test_autograd_function_matches_finite_differences():
     assert _rel(ge @ de, fde) < TOL, (ge @ de, fde)
     assert _rel(gt @ dth, fdt) < TOL, (gt @ dth, fdt)
     s.close()
+
+
+def test_batched_adjoint_matches_solo_bitwise():
+    """Members of uneven sizes and step counts (periodic, walls, inflow, a
+    partial strip), each with its own tape and gauges: the batched reverse
+    sweep gives every member's solo cotangents bit for bit, and a tape that
+    lacks a step's snapshot is refused."""
+    import dataclasses
+    import sgnx
+    from sgnx import cases
+    cs = [cases.gate(Nx=8, Ny=6), cases.solitary(Nx=32, Ny=16), cases.shoal(seed=3, Nx=48, Ny=24),
+          dataclasses.replace(cases.gate(Nx=37, Ny=11, bc=("periodic", "periodic", "wall", "wall")),
+                              name="gate37x11")]
+    steps = [7, 3, 5, 9]
+    rng = np.random.default_rng(77)
+    ss, cot, solo = [], [], []
+    for c, k in zip(cs, steps):
+        theta = c.depth["Hnod"] if c.depth is not None else np.zeros(c.nd)
+        nodes = rng.choice(c.nd, size=4, replace=False).astype(np.int64)
+        s, UN, G = _taped_run(c, theta, c.U0, nodes, k, 1 << 20)
+        ub, gb = rng.normal(size=UN.size), rng.normal(size=G.shape)
+        solo.append(s.adjoint(ub, gb))
+        cot.append((ub, gb))
+        ss.append(s)
+    b = sgnx.Batch(ss)
+    res = b.adjoint([u for u, _ in cot], [g for _, g in cot])
+    for i, ((U0b, thb), (sU, sT)) in enumerate(zip(res, solo)):
+        assert_bitwise(U0b, sU, f"member {i} ({cs[i].name}) U0bar")
+        assert_bitwise(thb, sT, f"member {i} ({cs[i].name}) thetaBar")
+    # a member whose tape was thinned is refused
+    c = cs[0]
+    ss[0].upload(c.U0)
+    ss[0].tapeBegin(2)
+    ss[0].integrateAdaptive(0.0, 1e9, ss[0].cflTimeStep(None, 0.5),
+                            sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=6))
+    ss[0].tapeEnd()
+    with pytest.raises(sgnx.SgnxError, match="every step"):
+        b.adjoint()
+    b.close()
+    for s in ss:
+        s.close()
