@@ -398,6 +398,108 @@ def inversion_leg(args):
     return out
 
 
+def batched_adjoint_leg(args):
+    """Batched adjoints: --adjoint-members members of the shoal sweep (seeded
+    widths 128-512, Ny = Nx/2: cases.sweep_member, inflow on the left, walls,
+    a noisy seeded shoal), each run --adjoint-member-steps accepted steps at
+    atol 1e-10, rtol 1e-8 with 8 gauges next to the inflow side and taped;
+    the gradient of the gauge
+    energy 0.5 |G|^2 with respect to the still depth (the initial state being
+    the fluid at rest, h = theta). The members' reverse sweeps run one after
+    another, then batched (one launch per kind over all members); the
+    cotangents must agree bit for bit. Each member's gradient is checked in one
+    smooth direction near its gauges against fourth-order differences of its
+    plain solver's replay."""
+    import numpy as np
+    import sgnx
+    from sgnx import cases
+    sizes = cases.sweep_sizes(args.adjoint_members, seed=args.seed + 5, nx_min=128, nx_max=512)
+    steps = args.adjoint_member_steps
+    members, data = [], []
+    t_fwd = 0.0
+    t_tape = 0.0
+    for i, nx in enumerate(sizes):
+        c = cases.sweep_member(args.seed * 1000 + 500 + i, nx)
+        theta = c.depth["Hnod"]
+        s = sgnx.SGN2D(**c.solver_kwargs())
+        s.setDepth(theta)
+        # gauges in the first two element columns, where the incident wave
+        # arrives within the run: node (a=2, b=2) of elements (ex, ey)
+        eys = np.linspace(0, c.Ny - 1, 4).astype(np.int64)
+        nodes = np.array([(ey * c.Nx + ex) * 25 + 12 for ey in eys for ex in (0, 1)],
+                         dtype=np.int64)
+        s.upload(c.U0)
+        dt0 = s.cflTimeStep(None, 0.5)
+        opts = sgnx.RkOpts(atol=1e-10, rtol=1e-8, max_accepted=steps)
+        t0 = time.perf_counter()
+        s.integrateAdaptive(0.0, 1e9, dt0, opts)
+        t_fwd += time.perf_counter() - t0
+        s.upload(c.U0)
+        s.setGauges(nodes, steps)
+        s.tapeBegin(steps + 1)
+        t0 = time.perf_counter()
+        s.integrateAdaptive(0.0, 1e9, dt0, opts)
+        t_tape += time.perf_counter() - t0
+        s.tapeEnd()
+        G = s.gaugeLog()[1]
+        members.append(s)
+        data.append((c, theta, nodes, G))
+    nodes_total = sum(s.nDof() for s in members)
+    # solo reverse sweeps, one after another (first a warm-up of each)
+    for s, (c, theta, nodes, G) in zip(members, data):
+        s.adjoint(None, G)
+    t0 = time.perf_counter()
+    solo = [s.adjoint(None, G) for s, (c, theta, nodes, G) in zip(members, data)]
+    t_seq = time.perf_counter() - t0
+    b = sgnx.Batch(members)
+    b.adjoint(None, [G for _, _, _, G in data])
+    t0 = time.perf_counter()
+    res = b.adjoint(None, [G for _, _, _, G in data])
+    t_bat = time.perf_counter() - t0
+    b.close()
+    same = all(np.array_equal(r[0].view(np.uint64), q[0].view(np.uint64)) and
+               np.array_equal(r[1].view(np.uint64), q[1].view(np.uint64))
+               for r, q in zip(res, solo))
+    # one smooth direction per member against its plain solver's replay
+    errs = []
+    rng = np.random.default_rng(args.seed + 55)
+    eps = 3e-3
+    for s, (c, theta, nodes, G), (U0b, thb) in zip(members, data, res):
+        g_theta = thb + U0b[:c.nd]                  # h0 = theta at rest
+        dth = cases.smooth_direction(c, rng, near=nodes)
+        rest = c.U0[c.nd:]
+
+        def energy(th):
+            s.setDepth(th)
+            s.upload(np.concatenate([th, rest]))
+            s.setGauges(nodes, steps)
+            s.tapeReplay()
+            return 0.5 * np.sum(s.gaugeLog()[1] ** 2)
+        f = {k: energy(theta + k * eps * dth) for k in (-2, -1, 1, 2)}
+        fd = (8.0 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12.0 * eps)
+        ad = g_theta @ dth
+        errs.append(abs(ad - fd) / max(abs(ad), abs(fd), 1e-300))
+    log("[bench] batched adjoint FD errors: " + " ".join(f"{e:.1e}" for e in errs))
+    for s in members:
+        s.close()
+    out = {"members": len(members), "nx_min": min(sizes), "nx_max": max(sizes),
+           "nodes": nodes_total, "steps_per_member": steps,
+           "forward_s": t_fwd, "taped_forward_s": t_tape,
+           "sequential_adjoint_s": t_seq, "batched_adjoint_s": t_bat,
+           "batched_speedup": t_seq / t_bat,
+           "adjoint_over_forward_sequential": t_seq / t_fwd,
+           "adjoint_over_forward_batched": t_bat / t_fwd,
+           "members_bit_identical": bool(same),
+           "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
+           "fd_reference": ("fourth-order central differences (eps 3e-3) of each member's plain "
+                            "replay along one smooth direction near its gauges")}
+    log(f"[bench] batched adjoint: {len(members)} members Nx {min(sizes)}..{max(sizes)}, "
+        f"{nodes_total / 1e6:.1f} M nodes, {steps} steps each; forward {t_fwd:.2f}s, "
+        f"reverse one by one {t_seq:.2f}s, batched {t_bat:.2f}s (x{t_seq / t_bat:.2f}), "
+        f"identical={same}; FD max rel err {max(errs):.2e}")
+    return out
+
+
 def cpu_baseline(case, gpu_solver, dt0, args):
     """The oracle on one core: FSAL head + 2 accepted steps of the same workload."""
     import numpy as np
@@ -453,6 +555,9 @@ def main():
                     help="accepted steps of the gauge-inversion gradient leg (0 = skip)")
     ap.add_argument("--inversion-dirs", type=int, default=32,
                     help="finite-difference directions checked on the inversion gradient")
+    ap.add_argument("--adjoint-members", type=int, default=16,
+                    help="members of the batched adjoint leg (0 = skip)")
+    ap.add_argument("--adjoint-member-steps", type=int, default=60)
     ap.add_argument("--profile-rhs", type=int, default=0)
     ap.add_argument("--profile-vjp", type=int, default=0,
                     help="profile mode: this many RHS vector-Jacobian products")
@@ -556,6 +661,14 @@ def main():
             inversion = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] inversion leg failed: {inversion['error']}")
 
+    batched_adjoint = None
+    if args.adjoint_members > 0:
+        try:
+            batched_adjoint = batched_adjoint_leg(args)
+        except Exception as e:                    # reported, not fatal to the main line
+            batched_adjoint = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] batched adjoint leg failed: {batched_adjoint['error']}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -601,6 +714,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "inversion": inversion,
+        "batched_adjoint": batched_adjoint,
     }
     print(json.dumps(out), flush=True)
 
