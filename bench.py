@@ -402,22 +402,20 @@ def batched_adjoint_leg(args):
     """Batched adjoints: --adjoint-members members of the shoal sweep (seeded
     widths 128-512, Ny = Nx/2: cases.sweep_member, inflow on the left, walls,
     a noisy seeded shoal), each run --adjoint-member-steps accepted steps at
-    atol 1e-10, rtol 1e-8 with 8 gauges next to the inflow side and taped;
-    the gradient of the gauge
-    energy 0.5 |G|^2 with respect to the still depth (the initial state being
-    the fluid at rest, h = theta). The members' reverse sweeps run one after
-    another, then batched (one launch per kind over all members); the
-    cotangents must agree bit for bit. Each member's gradient is checked in one
-    smooth direction near its gauges against fourth-order differences of its
-    plain solver's replay."""
+    atol 1e-10, rtol 1e-8 with 8 gauges next to the inflow side; the gradient
+    of the gauge energy 0.5 |G|^2 with respect to the still depth (the initial
+    state being the fluid at rest, h = theta). Timed one member after another
+    (plain run, taped run, reverse sweep) and as one batch (a batched taped
+    run, one launch per kind over all members, then the batched reverse
+    sweep); the cotangents must agree bit for bit. Each member's gradient is
+    checked in one smooth direction near its gauges against fourth-order
+    differences of its plain solver's replay."""
     import numpy as np
     import sgnx
     from sgnx import cases
     sizes = cases.sweep_sizes(args.adjoint_members, seed=args.seed + 5, nx_min=128, nx_max=512)
     steps = args.adjoint_member_steps
     members, data = [], []
-    t_fwd = 0.0
-    t_tape = 0.0
     for i, nx in enumerate(sizes):
         c = cases.sweep_member(args.seed * 1000 + 500 + i, nx)
         theta = c.depth["Hnod"]
@@ -430,32 +428,47 @@ def batched_adjoint_leg(args):
                          dtype=np.int64)
         s.upload(c.U0)
         dt0 = s.cflTimeStep(None, 0.5)
-        opts = sgnx.RkOpts(atol=1e-10, rtol=1e-8, max_accepted=steps)
-        t0 = time.perf_counter()
-        s.integrateAdaptive(0.0, 1e9, dt0, opts)
-        t_fwd += time.perf_counter() - t0
+        members.append(s)
+        data.append((c, theta, nodes, dt0))
+    opts = sgnx.RkOpts(atol=1e-10, rtol=1e-8, max_accepted=steps)
+
+    def start(s, c, nodes, taped):
         s.upload(c.U0)
         s.setGauges(nodes, steps)
-        s.tapeBegin(steps + 1)
-        t0 = time.perf_counter()
-        s.integrateAdaptive(0.0, 1e9, dt0, opts)
-        t_tape += time.perf_counter() - t0
-        s.tapeEnd()
-        G = s.gaugeLog()[1]
-        members.append(s)
-        data.append((c, theta, nodes, G))
-    nodes_total = sum(s.nDof() for s in members)
-    # solo reverse sweeps, one after another (first a warm-up of each)
-    for s, (c, theta, nodes, G) in zip(members, data):
-        s.adjoint(None, G)
+        if taped:
+            s.tapeBegin(steps + 1)
+
+    # one member after another
+    seq = {"forward": 0.0, "taped": 0.0, "reverse": 0.0}
+    for taped in (False, True):
+        for s, (c, theta, nodes, dt0) in zip(members, data):
+            start(s, c, nodes, taped)
+            t0 = time.perf_counter()
+            s.integrateAdaptive(0.0, 1e9, dt0, opts)
+            seq["taped" if taped else "forward"] += time.perf_counter() - t0
+            if taped:
+                s.tapeEnd()
+    Gs = [s.gaugeLog()[1] for s in members]
+    for s, G in zip(members, Gs):
+        s.adjoint(None, G)                         # warm-up
     t0 = time.perf_counter()
-    solo = [s.adjoint(None, G) for s, (c, theta, nodes, G) in zip(members, data)]
-    t_seq = time.perf_counter() - t0
+    solo = [s.adjoint(None, G) for s, G in zip(members, Gs)]
+    seq["reverse"] = time.perf_counter() - t0
+    # one batch: taped run and reverse sweep
     b = sgnx.Batch(members)
-    b.adjoint(None, [G for _, _, _, G in data])
+    bat = {}
+    for s, (c, theta, nodes, dt0) in zip(members, data):
+        start(s, c, nodes, True)
     t0 = time.perf_counter()
-    res = b.adjoint(None, [G for _, _, _, G in data])
-    t_bat = time.perf_counter() - t0
+    b.integrateAdaptive(0.0, 1e9, [d[3] for d in data], opts, graph=True)
+    bat["taped"] = time.perf_counter() - t0
+    for s in members:
+        s.tapeEnd()
+    Gb = [s.gaugeLog()[1] for s in members]
+    b.adjoint(None, Gb)                            # warm-up
+    t0 = time.perf_counter()
+    res = b.adjoint(None, Gb)
+    bat["reverse"] = time.perf_counter() - t0
     b.close()
     same = all(np.array_equal(r[0].view(np.uint64), q[0].view(np.uint64)) and
                np.array_equal(r[1].view(np.uint64), q[1].view(np.uint64))
@@ -464,7 +477,7 @@ def batched_adjoint_leg(args):
     errs = []
     rng = np.random.default_rng(args.seed + 55)
     eps = 3e-3
-    for s, (c, theta, nodes, G), (U0b, thb) in zip(members, data, res):
+    for s, (c, theta, nodes, dt0), (U0b, thb) in zip(members, data, res):
         g_theta = thb + U0b[:c.nd]                  # h0 = theta at rest
         dth = cases.smooth_direction(c, rng, near=nodes)
         rest = c.U0[c.nd:]
@@ -480,23 +493,28 @@ def batched_adjoint_leg(args):
         ad = g_theta @ dth
         errs.append(abs(ad - fd) / max(abs(ad), abs(fd), 1e-300))
     log("[bench] batched adjoint FD errors: " + " ".join(f"{e:.1e}" for e in errs))
+    nodes_total = sum(s.nDof() for s in members)
     for s in members:
         s.close()
+    grad_seq = seq["taped"] + seq["reverse"]
+    grad_bat = bat["taped"] + bat["reverse"]
     out = {"members": len(members), "nx_min": min(sizes), "nx_max": max(sizes),
            "nodes": nodes_total, "steps_per_member": steps,
-           "forward_s": t_fwd, "taped_forward_s": t_tape,
-           "sequential_adjoint_s": t_seq, "batched_adjoint_s": t_bat,
-           "batched_speedup": t_seq / t_bat,
-           "adjoint_over_forward_sequential": t_seq / t_fwd,
-           "adjoint_over_forward_batched": t_bat / t_fwd,
+           "sequential_s": seq, "batched_s": bat,
+           "gradient_sequential_s": grad_seq, "gradient_batched_s": grad_bat,
+           "gradient_speedup": grad_seq / grad_bat,
+           "reverse_speedup": seq["reverse"] / bat["reverse"],
+           "gradient_over_forward_sequential": grad_seq / seq["forward"],
+           "gradient_over_forward_batched": grad_bat / seq["forward"],
            "members_bit_identical": bool(same),
            "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
            "fd_reference": ("fourth-order central differences (eps 3e-3) of each member's plain "
                             "replay along one smooth direction near its gauges")}
     log(f"[bench] batched adjoint: {len(members)} members Nx {min(sizes)}..{max(sizes)}, "
-        f"{nodes_total / 1e6:.1f} M nodes, {steps} steps each; forward {t_fwd:.2f}s, "
-        f"reverse one by one {t_seq:.2f}s, batched {t_bat:.2f}s (x{t_seq / t_bat:.2f}), "
-        f"identical={same}; FD max rel err {max(errs):.2e}")
+        f"{nodes_total / 1e6:.1f} M nodes, {steps} steps each; one by one: run "
+        f"{seq['forward']:.2f}s, taped {seq['taped']:.2f}s, reverse {seq['reverse']:.2f}s; "
+        f"batched: taped {bat['taped']:.2f}s, reverse {bat['reverse']:.2f}s; gradient "
+        f"x{grad_seq / grad_bat:.2f}, identical={same}; FD max rel err {max(errs):.2e}")
     return out
 
 

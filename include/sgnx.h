@@ -244,7 +244,7 @@ int sgnx_set_depth(sgnx_solver* s, const double* theta, size_t n, int mem);
 int sgnx_rhs_vjp(sgnx_solver* s, const double* U, const double* W, double* Ubar,
                  double* theta_bar, size_t n, double t, int mem);
 /* Tape of adaptive runs: between tape_begin and tape_end, sgnx_integrate
- * records every accepted step (t, dt, gauge log row) and snapshots of step
+ * (and sgnx_batch_integrate, for each member) records every accepted step (t, dt, gauge log row) and snapshots of step
  * input states, at most max_snapshots held (every K-th step; K doubles when
  * the budget is full). tape_info: steps recorded, stride K, snapshots held.
  * tape_replay: the recorded steps (fixed t, dt) on the resident state, with

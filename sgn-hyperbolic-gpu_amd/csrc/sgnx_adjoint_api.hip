@@ -332,7 +332,7 @@ int adjoint_outputs(sgnx_solver* s, AdjTape* tp, double* U0bar, double* theta_ba
 
 bool tape_recording(const sgnx_solver* s) { return s->tape && s->tape->recording; }
 
-int tape_step_begin(sgnx_solver* s) {
+int tape_step_begin(sgnx_solver* s, hipStream_t st) {
   AdjTape* tp = s->tape;
   const long n = (long)tp->t.size();
   if (n % tp->K != 0 || (long)tp->snap.size() > n / tp->K) return SGNX_OK;
@@ -355,8 +355,7 @@ int tape_step_begin(sgnx_solver* s) {
     HIPCHK(s, hipMalloc((void**)&p, s->n() * sizeof(double)));
   }
   tp->snap.push_back(p);
-  HIPCHK(s, hipMemcpyAsync(p, s->d_U, s->n() * sizeof(double), hipMemcpyDeviceToDevice,
-                           s->stream));
+  HIPCHK(s, hipMemcpyAsync(p, s->d_U, s->n() * sizeof(double), hipMemcpyDeviceToDevice, st));
   return SGNX_OK;
 }
 

@@ -318,7 +318,7 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
   while (c.next()) {
     const double t = c.t, dt = c.dt;
     if (taping && stepBegun != c.S.accepted) {
-      if ((rc = tape_step_begin(s))) return rc;
+      if ((rc = tape_step_begin(s, s->stream))) return rc;
       stepBegun = c.S.accepted;
     }
     if (s->g.variant == 1) {

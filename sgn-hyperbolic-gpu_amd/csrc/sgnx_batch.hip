@@ -10,6 +10,10 @@
 // sgnx_integrate on it alone. The reference's counterpart is a parameter sweep
 // run as separate jobs (scripts/shoal_sweep.sbatch).
 //
+// Members recording an adjoint tape (sgnx_tape_begin) record it here as in
+// sgnx_integrate: snapshot copies on the batch stream ahead of a step, the
+// accepted steps after it.
+//
 // A step is: member arguments (pinned host -> device), three stage launches,
 // the fold, the (sum, max) pairs (device -> pinned host), one synchronisation;
 // optionally captured once into a hipGraph and replayed while the set of
@@ -284,6 +288,13 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
         b->h_mem[(size_t)(k + 1) * n + slot] = member_args(s, L[k].sa.u, L[k].dU, L[k].sa, L[k].t);
     }
     const long long total = fill_offsets(b, set);
+    // members recording a tape: the step's input state, on the batch stream
+    // ahead of the step (outside any captured graph)
+    for (int slot = 0; slot < A; ++slot) {
+      sgnx_solver* s = b->m[set[slot]];
+      if (tape_recording(s) && tape_step_begin(s, b->stream))
+        return bfail(b, SGNX_EDEVICE, "batch: tape snapshot: " + s->err);
+    }
     if (useGraph) {
       if (!b->exec || b->graphSet != set) {
         drop_graph(b);
@@ -309,10 +320,13 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
       const int i = set[slot];
       sgnx_solver* s = b->m[i];
       ctl[i].S.rhsEvals += 3;
+      const double tStep = ctl[i].t, dtStep = ctl[i].dt;
       if (ctl[i].finish(b->h_out[2 * slot], b->h_out[2 * slot + 1])) {
         std::swap(s->d_U, s->d_uNew);
         std::swap(s->d_k1, s->d_k4);
+        const long rows = s->grows;
         BCHK(b, record_gauges(s, b->stream, ctl[i].t));
+        if (tape_recording(s)) tape_accept(s, tStep, dtStep, s->grows > rows ? rows : -1);
       }
     }
   }

@@ -54,12 +54,13 @@ struct sgnx_solver {
   size_t n() const { return (size_t)NV * (size_t)g.Nd; }
 };
 
-// Adjoint tape hooks of the adaptive loop (sgnx_adjoint_api.hip): before the
-// first attempt of every step (snapshot of the step's input state when due),
+// Adjoint tape hooks of the adaptive loops (sgnx_adjoint_api.hip): before the
+// first attempt of every step (snapshot of the step's input state when due,
+// copied on stream st),
 // after every accepted step (its time, step size and gauge log row), and the
 // release of the tape's buffers.
 bool tape_recording(const sgnx_solver* s);
-int tape_step_begin(sgnx_solver* s);
+int tape_step_begin(sgnx_solver* s, hipStream_t st);
 void tape_accept(sgnx_solver* s, double t, double dt, long row);
 void tape_free(sgnx_solver* s);
 

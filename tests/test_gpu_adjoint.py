@@ -426,3 +426,45 @@ def test_batched_adjoint_matches_solo_bitwise():
     b.close()
     for s in ss:
         s.close()
+
+
+def test_batched_forward_records_the_solo_tapes():
+    """Members recording tapes inside a batched adaptive run (graph on) get the
+    tapes of their solo runs: the same steps, and batched and solo reverse
+    sweeps give the same cotangents bit for bit."""
+    import sgnx
+    from sgnx import cases
+    cs = [cases.gate(Nx=8, Ny=6), cases.solitary(Nx=32, Ny=16), cases.shoal(seed=3, Nx=48, Ny=24)]
+    steps = [6, 4, 5]
+    rng = np.random.default_rng(5)
+    solo, ss, cot = [], [], []
+    for c, k in zip(cs, steps):
+        nodes = rng.choice(c.nd, size=3, replace=False).astype(np.int64)
+        s, UN, G = _taped_run(c, c.depth["Hnod"], c.U0, nodes, k, 1 << 20)
+        ub, gb = rng.normal(size=UN.size), rng.normal(size=G.shape)
+        solo.append((s.tapeSteps(), s.adjoint(ub, gb)))
+        cot.append((ub, gb, nodes))
+        ss.append(s)
+    for c, s, (_, _, nodes) in zip(cs, ss, cot):
+        s.upload(c.U0)
+        s.setGauges(nodes, 64)
+        s.tapeBegin(1 << 20)
+    b = sgnx.Batch(ss)
+    dt0 = [s.cflTimeStep(None, 0.5) for s in ss]
+    for c, s in zip(cs, ss):
+        s.upload(c.U0)
+    opts = [sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=k) for k in steps]
+    b.integrateAdaptive(0.0, 1e9, dt0, opts, graph=True)
+    for s in ss:
+        s.tapeEnd()
+    for i, s in enumerate(ss):
+        tt, dts = s.tapeSteps()
+        assert_bitwise(tt, solo[i][0][0], f"member {i} step times")
+        assert_bitwise(dts, solo[i][0][1], f"member {i} step sizes")
+    res = b.adjoint([u for u, _, _ in cot], [g for _, g, _ in cot])
+    for i, (U0b, thb) in enumerate(res):
+        assert_bitwise(U0b, solo[i][1][0], f"member {i} U0bar")
+        assert_bitwise(thb, solo[i][1][1], f"member {i} thetaBar")
+    b.close()
+    for s in ss:
+        s.close()
