@@ -23,6 +23,7 @@
 #ifndef SGNX_SGN2D_HPP
 #define SGNX_SGN2D_HPP
 
+#include <algorithm>
 #include <cstddef>
 #include <functional>
 #include <stdexcept>
@@ -178,10 +179,58 @@ class SGN2DHip {
   void rk4Step(double t, double dt) { detail::check(sgnx_rk4_step(s_, t, dt), s_); }
   void bs3Step(double t, double dt) { detail::check(sgnx_bs3_step(s_, t, dt), s_); }
 
+  // ---- gauges and the discrete adjoint (no counterpart in the reference) ----
+  // eta = h - H at `nodes` after every accepted step, up to cap rows.
+  void setGauges(const std::vector<long long>& nodes, long cap) {
+    detail::check(sgnx_set_gauges(s_, nodes.data(), (int)nodes.size(), cap), s_);
+    nG_ = (int)nodes.size();
+    gcap_ = cap;
+  }
+  // the recorded rows, row-major (rows x gauges)
+  std::vector<double> gaugeLog() {
+    std::vector<double> eta((size_t)std::max(1L, gcap_) * std::max(1, nG_));
+    long rows = 0;
+    detail::check(sgnx_gauge_log(s_, eta.data(), nullptr, gcap_, &rows), s_);
+    eta.resize((size_t)rows * nG_);
+    return eta;
+  }
+  // still depth from the nodal field theta (face depths: the mean of the two
+  // edge values meeting at a line node)
+  void setDepth(const std::vector<double>& theta) {
+    detail::check(sgnx_set_depth(s_, theta.data(), theta.size(), SGNX_MEM_HOST), s_);
+  }
+  // Ubar = (dR/dU)^T W, thetaBar = (dR/dtheta)^T W (when given)
+  void rhsVjp(const State& U, const State& W, State& Ubar, std::vector<double>* thetaBar = nullptr,
+              double t = 0.0) {
+    Ubar.resize(U.size());
+    if (thetaBar) thetaBar->resize((size_t)nDof());
+    detail::check(sgnx_rhs_vjp(s_, U.data(), W.data(), Ubar.data(),
+                               thetaBar ? thetaBar->data() : nullptr, U.size(), t, SGNX_MEM_HOST),
+                  s_);
+  }
+  // record the following integrateAdaptive calls (at most maxSnapshots states)
+  void tapeBegin(long maxSnapshots) { detail::check(sgnx_tape_begin(s_, maxSnapshots), s_); }
+  void tapeEnd() { detail::check(sgnx_tape_end(s_), s_); }
+  void tapeSlots(long slots) { detail::check(sgnx_tape_set_slots(s_, slots), s_); }
+  // the recorded steps again (same t, dt) from the resident state
+  void tapeReplay() { detail::check(sgnx_tape_replay(s_), s_); }
+  // cotangents of the recorded run's initial state and still depth
+  void adjoint(const State* UbarFinal, const std::vector<double>* gaugeBar, State& U0bar,
+               std::vector<double>* thetaBar = nullptr) {
+    U0bar.resize((size_t)nTotal());
+    if (thetaBar) thetaBar->resize((size_t)nDof());
+    detail::check(sgnx_adjoint(s_, UbarFinal ? UbarFinal->data() : nullptr,
+                               gaugeBar ? gaugeBar->data() : nullptr, U0bar.data(),
+                               thetaBar ? thetaBar->data() : nullptr, SGNX_MEM_HOST),
+                  s_);
+  }
+
   sgnx_solver* handle() { return s_; }
 
  private:
   sgnx_solver* s_ = nullptr;
+  int nG_ = 0;
+  long gcap_ = 0;
 };
 
 // ---------------------------------------------------------------------------

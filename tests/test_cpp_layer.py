@@ -35,7 +35,7 @@ def test_constant_divisor_quotient_is_ieee_division():
     assert " 0 mismatches" in out.stdout
 
 
-@pytest.mark.parametrize("prog", ["dropin_sgnx", "dropin_ref"])
+@pytest.mark.parametrize("prog", ["dropin_sgnx", "adjoint_sgnx", "dropin_ref"])
 def test_cpp_layer_without_device_reports_error(prog):
     path = DROPIN_REF if prog == "dropin_ref" else _binary(prog)
     if not os.path.exists(path):
@@ -62,3 +62,12 @@ def test_sgnx_cpp_class_matches_oracle():
     out = subprocess.run([_binary("dropin_sgnx")], capture_output=True, text=True, timeout=300)
     print(out.stdout)
     assert out.returncode == 0 and "DROPIN_OK" in out.stdout, out.stdout + out.stderr
+
+
+@pytest.mark.gpu
+def test_sgnx_cpp_adjoint_matches_differences():
+    """sgnx::SGN2DHip's gauges, tape and reverse sweep: the directional
+    derivative of a taped run against fourth-order differences of its replay."""
+    out = subprocess.run([_binary("adjoint_sgnx")], capture_output=True, text=True, timeout=300)
+    print(out.stdout)
+    assert out.returncode == 0 and "ADJOINT_OK" in out.stdout, out.stdout + out.stderr
