@@ -223,11 +223,13 @@ def adjoint_leg(args):
     """Cost of a gradient on the timed workload: an adaptive run of
     --adjoint-steps accepted steps from the shoal state, its still depth set
     from the nodal field theta (SGN2D.setDepth), gauges at 16 nodes; then the
-    same run taped (every step's input state kept) and the reverse sweep for
-    random cotangents of the final state and of the gauge log, giving the
-    cotangents of the initial state and of theta. Reports the sweep and the
-    whole gradient (taped run + sweep) against the plain run, and one RHS
-    vector-Jacobian product against one RHS."""
+    same run taped (every step's input state kept, and its stage inputs, which
+    the forward's stage launches write) and the reverse sweep for random
+    cotangents of the final state and of the gauge log, giving the cotangents
+    of the initial state and of theta. Reports the sweep and the whole
+    gradient (taped run + sweep) against the plain run, the same for a tape
+    without stages (the sweep recomputes them), and one RHS vector-Jacobian
+    product against one RHS."""
     import numpy as np
     import sgnx
     from sgnx import cases, hip
@@ -244,10 +246,11 @@ def adjoint_leg(args):
     Ubar = rng.normal(size=n)
     Gbar = rng.normal(size=(steps, nodes.size))
 
-    def run(taped):
+    def run(taped, keep=False):
         s.upload(case.U0)
         s.setGauges(nodes, steps)
         if taped:
+            s.tapeKeepStages(keep)
             s.tapeBegin(steps + 1)
         t0 = time.perf_counter()
         st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
@@ -256,17 +259,29 @@ def adjoint_leg(args):
             s.tapeEnd()
         return st, wall
 
-    run(True)                                      # warm-up: allocates the tape
+    run(True, False)                               # warm-up: allocates the tape
     s.adjoint(Ubar, Gbar)
     st0, t_fwd = run(False)
-    st1, t_tape = run(True)
     Ud = hip.DeviceArray.from_host(Ubar)
     Gd = hip.DeviceArray.from_host(Gbar)
     U0b = hip.DeviceArray(n)
     thb = hip.DeviceArray(nd)
-    t0 = time.perf_counter()
-    s.adjoint_device(Ud.address, Gd.address, U0b.address, thb.address)
-    t_adj = time.perf_counter() - t0
+
+    def sweep():
+        t0 = time.perf_counter()
+        s.adjoint_device(Ud.address, Gd.address, U0b.address, thb.address)
+        return time.perf_counter() - t0, U0b.download(), thb.download()
+
+    # the sweep recomputing each step's stages, then with the stages kept
+    # (after a warm-up run that allocates their buffers)
+    st1, t_tape_r = run(True, False)
+    t_adj_r, U0b_r, thb_r = sweep()
+    run(True, True)
+    st2, t_tape = run(True, True)
+    kept = s.tapeStages()
+    t_adj, U0b_k, thb_k = sweep()
+    same = bool(np.array_equal(U0b_r.view(np.int64), U0b_k.view(np.int64))
+                and np.array_equal(thb_r.view(np.int64), thb_k.view(np.int64)))
     # one product against one RHS on the final state (device operands)
     U = hip.DeviceArray.from_host(s.download())
     W = hip.DeviceArray.from_host(rng.normal(size=n))
@@ -284,21 +299,26 @@ def adjoint_leg(args):
         s.rhs_device(U.address, dU.address)
     hip.synchronize()
     t_rhs = (time.perf_counter() - t0) / reps
-    ok = bool(np.all(np.isfinite(U0b.download())) and np.all(np.isfinite(thb.download())))
+    ok = bool(np.all(np.isfinite(U0b_k)) and np.all(np.isfinite(thb_k)))
     info = s.tapeInfo()
     s.close()
     out = {"workload": f"shoal {case.Nx}x{case.Ny}, {steps} accepted steps, 16 gauges",
-           "same_step_counts": st0.as_dict() == st1.as_dict(),
+           "same_step_counts": st0.as_dict() == st1.as_dict() == st2.as_dict(),
            "accepted": st0.accepted, "rejected": st0.rejected,
            "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "adjoint_ms": 1e3 * t_adj,
            "adjoint_over_forward": t_adj / t_fwd,
            "gradient_over_forward": (t_tape + t_adj) / t_fwd,
+           "stages_kept": kept,
+           "recomputing": {"taped_forward_ms": 1e3 * t_tape_r, "adjoint_ms": 1e3 * t_adj_r,
+                           "gradient_over_forward": (t_tape_r + t_adj_r) / t_fwd},
+           "kept_equals_recomputed": same,
            "snapshots": info[2], "stride": info[1],
            "vjp_ms": 1e3 * t_vjp, "rhs_ms": 1e3 * t_rhs, "vjp_over_rhs": t_vjp / t_rhs,
            "cotangents_finite": ok}
     log(f"[bench] adjoint: forward {1e3 * t_fwd:.1f} ms, taped {1e3 * t_tape:.1f} ms, "
         f"reverse sweep {1e3 * t_adj:.1f} ms -> sweep/forward {t_adj / t_fwd:.2f}, "
-        f"gradient/forward {(t_tape + t_adj) / t_fwd:.2f}; VJP {1e3 * t_vjp:.3f} ms vs RHS "
+        f"gradient/forward {(t_tape + t_adj) / t_fwd:.2f} (stages kept; recomputing "
+        f"{(t_tape_r + t_adj_r) / t_fwd:.2f}, same bits {same}); VJP {1e3 * t_vjp:.3f} ms vs RHS "
         f"{1e3 * t_rhs:.3f} ms")
     return out
 
@@ -407,7 +427,8 @@ def batched_adjoint_leg(args):
     state being the fluid at rest, h = theta). Timed one member after another
     (plain run, taped run, reverse sweep) and as one batch (a batched taped
     run, one launch per kind over all members, then the batched reverse
-    sweep); the cotangents must agree bit for bit. Each member's gradient is
+    sweep); the tapes keep the stage inputs; the cotangents must agree bit
+    for bit. Each member's gradient is
     checked in one smooth direction near its gauges against fourth-order
     differences of its plain solver's replay."""
     import numpy as np
@@ -436,6 +457,7 @@ def batched_adjoint_leg(args):
         s.upload(c.U0)
         s.setGauges(nodes, steps)
         if taped:
+            s.tapeKeepStages(True)
             s.tapeBegin(steps + 1)
 
     # one member after another

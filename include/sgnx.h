@@ -255,6 +255,15 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots);
  * schedule (Griewank's revolve bound) inside each segment, plus two states of
  * scratch; the result is the same bit for bit. */
 int sgnx_tape_set_slots(sgnx_solver* s, long slots);
+/* on != 0 (set before the run; set before sgnx_tape_begin, it also keeps
+ * the buffers of an earlier run for reuse): the taped run of the fused BS3 path also
+ * keeps each accepted step's stage-2 and stage-3 inputs, which its stage
+ * launches write as they form them (two more states per step), so the reverse
+ * sweep recomputes no stage. They are dropped, and the sweep recomputes, once
+ * the snapshot budget is full. tape_stages: the steps whose stages are kept.
+ * The adjoint is the same bit for bit either way. */
+int sgnx_tape_keep_stages(sgnx_solver* s, int on);
+int sgnx_tape_stages(const sgnx_solver* s, long* steps);
 /* of the last sgnx_adjoint: recorded steps it recomputed, slots it held at once */
 int sgnx_adjoint_stats(const sgnx_solver* s, long* recomputed_steps, long* peak_slots);
 int sgnx_tape_end(sgnx_solver* s);

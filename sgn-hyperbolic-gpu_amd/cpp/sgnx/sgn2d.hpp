@@ -212,6 +212,9 @@ class SGN2DHip {
   void tapeBegin(long maxSnapshots) { detail::check(sgnx_tape_begin(s_, maxSnapshots), s_); }
   void tapeEnd() { detail::check(sgnx_tape_end(s_), s_); }
   void tapeSlots(long slots) { detail::check(sgnx_tape_set_slots(s_, slots), s_); }
+  // keep each taped step's stage inputs (two more states per step) so the
+  // reverse sweep recomputes no stage; set before the run
+  void tapeKeepStages(bool on) { detail::check(sgnx_tape_keep_stages(s_, on ? 1 : 0), s_); }
   // the recorded steps again (same t, dt) from the resident state
   void tapeReplay() { detail::check(sgnx_tape_replay(s_), s_); }
   // cotangents of the recorded run's initial state and still depth

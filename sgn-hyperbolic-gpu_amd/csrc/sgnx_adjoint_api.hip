@@ -136,6 +136,21 @@ int step_adjoint(sgnx_solver* s, AdjTape* tp, const double* u, double t, double 
   return vjp(s, a1, t);
 }
 
+// step_adjoint with the stage inputs y2, y3 the taped run kept
+int step_adjoint_stored(sgnx_solver* s, AdjTape* tp, const double* u, const double* y2,
+                        const double* y3, double t, double dt, double* thetaBar) {
+  int rc;
+  VjpArgs a3 = vjp_args(tp, y3, tp->lam, dt * (4.0 / 9.0), nullptr, 0.0, tp->y3b, thetaBar);
+  if ((rc = vjp(s, a3, t + 0.75 * dt))) return rc;
+  VjpArgs a2 = vjp_args(tp, y2, tp->lam, dt * (1.0 / 3.0), tp->y3b, 0.75 * dt, tp->y2b, thetaBar);
+  if ((rc = vjp(s, a2, t + 0.5 * dt))) return rc;
+  VjpArgs a1 = vjp_args(tp, u, tp->lam, dt * (2.0 / 9.0), tp->y2b, 0.5 * dt, tp->lam, thetaBar);
+  a1.acc0 = tp->lam;
+  a1.acc1 = tp->y3b;
+  a1.acc2 = tp->y2b;
+  return vjp(s, a1, t);
+}
+
 int ensure_vjp_buffers(sgnx_solver* s, AdjTape* tp) {
   int rc;
   if ((rc = alloc_state(s, &tp->coef, (size_t)vjp_coef_doubles(s->g)))) return rc;
@@ -218,6 +233,8 @@ int reverse_step(sgnx_solver* s, AdjTape* tp, const double* u, long m, const dou
   if (gb && tp->row[m] >= 0 && tp->row[m] < s->grows)
     HIPCHK(s, launch_gauge_bar(gb + (size_t)tp->row[m] * s->nG, s->d_gnodes, s->nG, tp->lam,
                                tp->thb, s->stream));
+  if (m < (long)tp->y2s.size())
+    return step_adjoint_stored(s, tp, u, tp->y2s[m], tp->y3s[m], tp->t[m], tp->dt[m], tp->thb);
   return step_adjoint(s, tp, u, tp->t[m], tp->dt[m], tp->thb);
 }
 
@@ -270,6 +287,23 @@ AdjTape* tape_of(sgnx_solver* s) {
 void release_snapshots(AdjTape* tp) {
   for (double* p : tp->snap) tp->spare.push_back(p);
   tp->snap.clear();
+}
+
+void release_stages(AdjTape* tp) {
+  for (double* p : tp->y2s) tp->spare.push_back(p);
+  for (double* p : tp->y3s) tp->spare.push_back(p);
+  tp->y2s.clear();
+  tp->y3s.clear();
+}
+
+int take_spare(sgnx_solver* s, AdjTape* tp, double** out) {
+  if (!tp->spare.empty()) {
+    *out = tp->spare.back();
+    tp->spare.pop_back();
+    return SGNX_OK;
+  }
+  HIPCHK(s, hipMalloc((void**)out, s->n() * sizeof(double)));
+  return SGNX_OK;
 }
 
 }  // namespace
@@ -332,11 +366,31 @@ int adjoint_outputs(sgnx_solver* s, AdjTape* tp, double* U0bar, double* theta_ba
 
 bool tape_recording(const sgnx_solver* s) { return s->tape && s->tape->recording; }
 
+bool tape_stage_outputs(const sgnx_solver* s, double** y2, double** y3) {
+  const AdjTape* tp = s->tape;
+  if (!tp || !tp->recording || !tp->keepStages || tp->stagesOff) return false;
+  if (tp->y2s.size() != tp->t.size() + 1) return false;
+  *y2 = tp->y2s.back();
+  *y3 = tp->y3s.back();
+  return true;
+}
+
 int tape_step_begin(sgnx_solver* s, hipStream_t st) {
   AdjTape* tp = s->tape;
   const long n = (long)tp->t.size();
+  int rc;
+  if (tp->keepStages && !tp->stagesOff && s->g.variant == 1 && (long)tp->y2s.size() == n) {
+    double *a = nullptr, *b = nullptr;
+    if ((rc = take_spare(s, tp, &a))) return rc;
+    if ((rc = take_spare(s, tp, &b))) return rc;
+    tp->y2s.push_back(a);
+    tp->y3s.push_back(b);
+  }
   if (n % tp->K != 0 || (long)tp->snap.size() > n / tp->K) return SGNX_OK;
   while ((long)tp->snap.size() >= tp->maxSnap) {
+    // the stage states are kept only while every step's snapshot is
+    tp->stagesOff = true;
+    release_stages(tp);
     // budget full: keep every other snapshot, double the stride
     std::vector<double*> keep;
     for (size_t j = 0; j < tp->snap.size(); ++j) {
@@ -348,12 +402,7 @@ int tape_step_begin(sgnx_solver* s, hipStream_t st) {
     if (n % tp->K != 0) return SGNX_OK;
   }
   double* p = nullptr;
-  if (!tp->spare.empty()) {
-    p = tp->spare.back();
-    tp->spare.pop_back();
-  } else {
-    HIPCHK(s, hipMalloc((void**)&p, s->n() * sizeof(double)));
-  }
+  if ((rc = take_spare(s, tp, &p))) return rc;
   tp->snap.push_back(p);
   HIPCHK(s, hipMemcpyAsync(p, s->d_U, s->n() * sizeof(double), hipMemcpyDeviceToDevice, st));
   return SGNX_OK;
@@ -370,6 +419,7 @@ void tape_free(sgnx_solver* s) {
   AdjTape* tp = s->tape;
   if (!tp) return;
   release_snapshots(tp);
+  release_stages(tp);
   for (double* p : tp->spare) (void)hipFree(p);
   for (double* p : tp->seg) (void)hipFree(p);
   for (double* p : {tp->lam, tp->y3b, tp->y2b, tp->coef, tp->thb, tp->gb, tp->pp[0], tp->pp[1]})
@@ -439,8 +489,12 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots) {
   if (!tp) return fail(s, SGNX_ENOMEM, "host allocation failed");
   HIPCHK(s, hipStreamSynchronize(s->stream));
   release_snapshots(tp);
-  // buffers beyond the new budget are returned to the device
-  while ((long)tp->spare.size() > max_snapshots) {
+  release_stages(tp);
+  tp->stagesOff = false;
+  // buffers beyond the new budget (three states a step while stages are
+  // kept) are returned to the device
+  const long cap = tp->keepStages ? 3 * max_snapshots : max_snapshots;
+  while ((long)tp->spare.size() > cap) {
     (void)hipFree(tp->spare.back());
     tp->spare.pop_back();
   }
@@ -460,6 +514,21 @@ int sgnx_tape_set_slots(sgnx_solver* s, long slots) {
   AdjTape* tp = tape_of(s);
   if (!tp) return fail(s, SGNX_ENOMEM, "host allocation failed");
   tp->slots = slots;
+  return SGNX_OK;
+}
+
+int sgnx_tape_keep_stages(sgnx_solver* s, int on) {
+  if (!s) return SGNX_EINVAL;
+  AdjTape* tp = tape_of(s);
+  if (!tp) return fail(s, SGNX_ENOMEM, "host allocation failed");
+  tp->keepStages = on != 0;
+  return SGNX_OK;
+}
+
+int sgnx_tape_stages(const sgnx_solver* s, long* steps) {
+  if (!s || !steps) return SGNX_EINVAL;
+  const AdjTape* tp = s->tape;
+  *steps = tp ? std::min((long)tp->y2s.size(), (long)tp->t.size()) : 0;
   return SGNX_OK;
 }
 

@@ -48,6 +48,7 @@ struct sgnx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int> graphSet;
+  bool graphKeep = false;   // the captured step keeps stage inputs
   // argument arena of the batched reverse sweep (pinned host, device)
   char* h_adj = nullptr;
   char* d_adj = nullptr;
@@ -107,8 +108,9 @@ long long fill_offsets(sgnx_batch* b, const std::vector<int>& set) {
 }
 
 // the device work of one attempted step for the members in `set` (slots
-// 0..A-1 of kinds 1..3 of h_mem filled by the caller)
-int enqueue_step(sgnx_batch* b, const std::vector<int>& set, long long total) {
+// 0..A-1 of kinds 1..3 of h_mem filled by the caller); `keep`: stages 2 and 3
+// also write their inputs where a member's stage names a buffer for them
+int enqueue_step(sgnx_batch* b, const std::vector<int>& set, long long total, bool keep) {
   const int A = (int)set.size();
   const size_t n = (size_t)b->n;
   BCHK(b, hipMemcpyAsync(b->d_mem, b->h_mem, 4 * n * sizeof(BatchMember),
@@ -117,7 +119,7 @@ int enqueue_step(sgnx_batch* b, const std::vector<int>& set, long long total) {
                          b->stream));
   for (int k = 1; k <= 3; ++k)
     BCHK(b, launch_rhs_stage_batch(b->d_mem + k * n, b->d_ts, A, total, k,
-                                   (b->waves3 >> k) & 1, b->stream));
+                                   (b->waves3 >> k) & 1, b->stream, keep && k < 3));
   BCHK(b, launch_finish_batch(b->d_mem + 3 * n, A, b->d_out, b->stream));
   BCHK(b, hipMemcpyAsync(b->h_out, b->d_out, 2 * (size_t)A * sizeof(double),
                          hipMemcpyDeviceToHost, b->stream));
@@ -279,27 +281,30 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
       if (ctl[i].next()) set.push_back(i);
     if (set.empty()) break;
     const int A = (int)set.size();
-    for (int slot = 0; slot < A; ++slot) {
-      const int i = set[slot];
-      sgnx_solver* s = b->m[i];
-      BS3Launch L[3];
-      bs3_launches(s, ctl[i].t, ctl[i].dt, opts[i].atol, opts[i].rtol, L);
-      for (int k = 0; k < 3; ++k)
-        b->h_mem[(size_t)(k + 1) * n + slot] = member_args(s, L[k].sa.u, L[k].dU, L[k].sa, L[k].t);
-    }
-    const long long total = fill_offsets(b, set);
     // members recording a tape: the step's input state, on the batch stream
-    // ahead of the step (outside any captured graph)
+    // ahead of the step (outside any captured graph), and the buffers its
+    // stage launches keep their inputs in
     for (int slot = 0; slot < A; ++slot) {
       sgnx_solver* s = b->m[set[slot]];
       if (tape_recording(s) && tape_step_begin(s, b->stream))
         return bfail(b, SGNX_EDEVICE, "batch: tape snapshot: " + s->err);
     }
+    bool keep = false;
+    for (int slot = 0; slot < A; ++slot) {
+      const int i = set[slot];
+      sgnx_solver* s = b->m[i];
+      BS3Launch L[3];
+      bs3_launches(s, ctl[i].t, ctl[i].dt, opts[i].atol, opts[i].rtol, L);
+      keep = keep || L[0].sa.yOut != nullptr;
+      for (int k = 0; k < 3; ++k)
+        b->h_mem[(size_t)(k + 1) * n + slot] = member_args(s, L[k].sa.u, L[k].dU, L[k].sa, L[k].t);
+    }
+    const long long total = fill_offsets(b, set);
     if (useGraph) {
-      if (!b->exec || b->graphSet != set) {
+      if (!b->exec || b->graphSet != set || b->graphKeep != keep) {
         drop_graph(b);
         BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        rc = enqueue_step(b, set, total);
+        rc = enqueue_step(b, set, total, keep);
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(b->stream, &g);
         if (rc) {
@@ -310,10 +315,11 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
         b->graph = g;
         BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->graphSet = set;
+        b->graphKeep = keep;
       }
       BCHK(b, hipGraphLaunch(b->exec, b->stream));
     } else {
-      if ((rc = enqueue_step(b, set, total))) return rc;
+      if ((rc = enqueue_step(b, set, total, keep))) return rc;
     }
     BCHK(b, hipStreamSynchronize(b->stream));
     for (int slot = 0; slot < A; ++slot) {
@@ -552,7 +558,7 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
   auto* hVT = reinterpret_cast<long long*>(b->h_adj + oVT);
   auto dptr = [&](size_t off) { return b->d_adj + off; };
   for (long k = 0; k < Nmax; ++k) {
-    int A = 0;
+    int A = 0, Rn = 0;   // members in this step; of those, the ones that recompute stages
     long long rAcc = 0, cAcc = 0, vAcc = 0;
     for (int i = 0; i < n; ++i) {
       AdjTape* T = tp[i];
@@ -571,28 +577,35 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
         gq.thetaBar = T->thb;
       }
       hG[A] = gq;
-      Stage s1{};
-      s1.u = u;
-      hR1[A] = member_args(s, u, s->d_k1, s1, t);
-      Stage s2{};
-      s2.u = s->d_tmp;
-      hR2[A] = member_args(s, s->d_tmp, s->d_k2, s2, t + 0.5 * dt);
-      hRT[A] = rAcc;
-      rAcc += sweep_tiles(s->g);
-      hC2[A] = CombMember{s->d_tmp, u, 0.5 * dt, s->d_k1, nst};
-      hC3[A] = CombMember{s->d_k3, u, 0.75 * dt, s->d_k2, nst};
-      hCT[A] = cAcc;
-      cAcc += comb_blocks(nst);
+      // stage inputs y2, y3: kept by the taped run, or recomputed here
+      const bool kept = m < (long)T->y2s.size();
+      const double* y2 = kept ? T->y2s[m] : s->d_tmp;
+      const double* y3 = kept ? T->y3s[m] : s->d_k3;
+      if (!kept) {
+        Stage s1{};
+        s1.u = u;
+        hR1[Rn] = member_args(s, u, s->d_k1, s1, t);
+        Stage s2{};
+        s2.u = s->d_tmp;
+        hR2[Rn] = member_args(s, s->d_tmp, s->d_k2, s2, t + 0.5 * dt);
+        hRT[Rn] = rAcc;
+        rAcc += sweep_tiles(s->g);
+        hC2[Rn] = CombMember{s->d_tmp, u, 0.5 * dt, s->d_k1, nst};
+        hC3[Rn] = CombMember{s->d_k3, u, 0.75 * dt, s->d_k2, nst};
+        hCT[Rn] = cAcc;
+        cAcc += comb_blocks(nst);
+        ++Rn;
+      }
       // uNew = u + dt ((2/9 k1 + 1/3 k2) + 4/9 k3)   (rk.cpp:106-108)
       VjpArgs a3{};
-      a3.U = s->d_k3;
+      a3.U = y3;
       a3.A = T->lam;
       a3.ca = dt * (4.0 / 9.0);
       a3.out = T->y3b;
       a3.thetaBar = T->thb;
       a3.coef = T->coef;
       VjpArgs a2 = a3;
-      a2.U = s->d_tmp;
+      a2.U = y2;
       a2.ca = dt * (1.0 / 3.0);
       a2.B = T->y3b;
       a2.cb = 0.75 * dt;
@@ -613,8 +626,8 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
       vAcc += vjp_tiles(s->g);
       ++A;
     }
-    hRT[A] = rAcc;
-    hCT[A] = cAcc;
+    hRT[Rn] = rAcc;
+    hCT[Rn] = cAcc;
     hVT[A] = vAcc;
     BCHK(b, hipMemcpyAsync(b->d_adj, b->h_adj, total, hipMemcpyHostToDevice, b->stream));
     const auto* dR1 = reinterpret_cast<const BatchMember*>(dptr(oR1));
@@ -623,12 +636,14 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
     const auto* dCT = reinterpret_cast<const long long*>(dptr(oCT));
     const auto* dVT = reinterpret_cast<const long long*>(dptr(oVT));
     BCHK(b, launch_gauge_bar_batch(reinterpret_cast<const GaugeMember*>(dptr(oG)), A, b->stream));
-    BCHK(b, launch_rhs_stage_batch(dR1, dRT, A, rAcc, kPlain, b->waves3 & 1, b->stream));
-    BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC2)), dCT, A, cAcc,
-                              b->stream));
-    BCHK(b, launch_rhs_stage_batch(dR2, dRT, A, rAcc, kPlain, b->waves3 & 1, b->stream));
-    BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC3)), dCT, A, cAcc,
-                              b->stream));
+    if (Rn > 0) {
+      BCHK(b, launch_rhs_stage_batch(dR1, dRT, Rn, rAcc, kPlain, b->waves3 & 1, b->stream));
+      BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC2)), dCT, Rn, cAcc,
+                                b->stream));
+      BCHK(b, launch_rhs_stage_batch(dR2, dRT, Rn, rAcc, kPlain, b->waves3 & 1, b->stream));
+      BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC3)), dCT, Rn, cAcc,
+                                b->stream));
+    }
     for (size_t o : {oV3, oV2, oV1})
       BCHK(b, launch_rhs_vjp_batch(reinterpret_cast<const VjpMember*>(dptr(o)), dVT, A, vAcc,
                                    b->stream));

@@ -60,6 +60,9 @@ struct sgnx_solver {
 // after every accepted step (its time, step size and gauge log row), and the
 // release of the tape's buffers.
 bool tape_recording(const sgnx_solver* s);
+// While a tape keeps stage states: the buffers the current step's stage 2 and
+// stage 3 launches write their stage inputs to (false: none).
+bool tape_stage_outputs(const sgnx_solver* s, double** y2, double** y3);
 int tape_step_begin(sgnx_solver* s, hipStream_t st);
 void tape_accept(sgnx_solver* s, double t, double dt, long row);
 void tape_free(sgnx_solver* s);
@@ -119,6 +122,11 @@ inline void bs3_launches(const sgnx_solver* s, double t, double dt, double atol,
   c.rtol = rtol;
   c.part = s->d_part;
   L[2] = {kErr, c, s->d_k4, t + dt};
+  double *y2 = nullptr, *y3 = nullptr;
+  if (tape_stage_outputs(s, &y2, &y3)) {
+    L[0].sa.yOut = y2;
+    L[1].sa.yOut = y3;
+  }
 }
 
 // The step-size controller of rk.cpp:67-164 for one run: the loop condition

@@ -85,6 +85,7 @@ struct Stage {
   const double* z;    // kErr: partial embedded sum
   double atol, rtol;  // kErr
   double* part;       // kErr
+  double* yOut;       // kAxpy, kCand (optional): the stage input u + a*k1 out
 };
 
 // One grid of a batched launch: its operands, geometry, constants, inflow
@@ -108,7 +109,8 @@ hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double
                       GhostTime gt, hipStream_t st);
 // dU = R(stage input) with the row-sweep kernel (kCand writes sa.uNew and
 // sa.zOut instead of dU; kErr also writes sa.part, 3 doubles per workgroup,
-// folded by launch_finish).
+// folded by launch_finish). kAxpy and kCand with sa.yOut set also write the
+// stage input they form (the batched form: keep = some member's yOut is set).
 hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const Stage& sa,
                             double* dU, GhostTime gt, hipStream_t st);
 // out[0] = sum, out[1] = max of nb (sum, carry, max) partial triples, fixed order
@@ -117,7 +119,8 @@ hipError_t launch_finish(const double* part, long long nb, double* out, hipStrea
 // mem[nMem]; tileStart[m] = sum of the tiles of members before m, totalTiles
 // their sum); `mode` as launch_rhs_stage, for all members.
 hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileStart, int nMem,
-                                  long long totalTiles, int mode, bool waves3, hipStream_t st);
+                                  long long totalTiles, int mode, bool waves3, hipStream_t st,
+                                  bool keep = false);
 // launch_finish for every member: member m's partials (mem[m].sa.part, its
 // nTiles) into out[2m], out[2m + 1]
 hipError_t launch_finish_batch(const BatchMember* mem, int nMem, double* out, hipStream_t st);

@@ -377,7 +377,7 @@ __device__ __forceinline__ unsigned node_of(int Nx, int ex, int ey, int a, int b
 
 // One workgroup's strip x chunk `tile` of one grid (the body of the plain and
 // the batched launches).
-template <int TW, int MODE, int WAVES>
+template <int TW, int MODE, int WAVES, bool KEEP = false>
 __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double* __restrict__ dU,
                                            const Geometry& g, const Phys& ph, const GhostTime& gt,
                                            const Stage& sa, int nStrips, int tile) {
@@ -571,6 +571,11 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     if (isNode) {
       double Un[NV];
       stage_combine<MODE>(pR, sa.a, Un);
+      if (KEEP && sa.yOut) {
+        // the stage input, kept for a later reverse sweep
+#pragma unroll
+        for (int v = 0; v < NV; ++v) (sa.yOut + (long long)v * Nd)[rowBase + t] = Un[v];
+      }
       const double Hn = pH, Hxn = pHx, Hyn = pHy;
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -775,18 +780,19 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
   }
 }
 
-template <int TW, int MODE, int WAVES>
+template <int TW, int MODE, int WAVES, bool KEEP = false>
 __global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
 k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph,
             GhostTime gt, Stage sa, int nStrips) {
-  sweep_tile<TW, MODE, WAVES>(U, dU, g, ph, gt, sa, nStrips, xcd_tile(blockIdx.x, gridDim.x));
+  sweep_tile<TW, MODE, WAVES, KEEP>(U, dU, g, ph, gt, sa, nStrips,
+                                    xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Many grids in one launch: the workgroups of every member's tiles, packed
 // back to back (tileStart[m] = first packed index of member m, ascending);
 // each member keeps its own tile decomposition, so its error partials, and
 // everything else, equal its own launch's.
-template <int TW, int MODE, int WAVES>
+template <int TW, int MODE, int WAVES, bool KEEP = false>
 __global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
 k_rhs_sweep_batch(const BatchMember* __restrict__ mem, const long long* __restrict__ tileStart,
                   int nMem) {
@@ -797,8 +803,8 @@ k_rhs_sweep_batch(const BatchMember* __restrict__ mem, const long long* __restri
     if (tileStart[mid] <= w) lo = mid; else hi = mid - 1;
   }
   const BatchMember& M = mem[lo];
-  sweep_tile<TW, MODE, WAVES>(M.U, M.dU, M.g, M.ph, M.gt, M.sa, M.nStrips,
-                              (int)(w - tileStart[lo]));
+  sweep_tile<TW, MODE, WAVES, KEEP>(M.U, M.dU, M.g, M.ph, M.gt, M.sa, M.nStrips,
+                                    (int)(w - tileStart[lo]));
 }
 
 // Hx, Hy at every node (sgn2d.cpp:54-71).
@@ -1086,12 +1092,18 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
   // g.waves3 set: 3 waves, else 2)
   const bool w3 = (g.waves3 >> mode) & 1;
   if (mode < kPlain || mode > kErr) return hipErrorInvalidValue;
-#define SGNX_SWEEP(M, W) \
-  hipLaunchKernelGGL((k_rhs_sweep<kTileW, M, W>), grid, block, 0, st, sa.u, dU, g, ph, gt, sa, strips)
-  if (mode == kPlain) { if (w3) SGNX_SWEEP(kPlain, 3); else SGNX_SWEEP(kPlain, 2); }
-  else if (mode == kAxpy) { if (w3) SGNX_SWEEP(kAxpy, 3); else SGNX_SWEEP(kAxpy, 2); }
-  else if (mode == kCand) { if (w3) SGNX_SWEEP(kCand, 3); else SGNX_SWEEP(kCand, 2); }
-  else { if (w3) SGNX_SWEEP(kErr, 3); else SGNX_SWEEP(kErr, 2); }
+#define SGNX_SWEEP(M, W, K)                                                             \
+  hipLaunchKernelGGL((k_rhs_sweep<kTileW, M, W, K>), grid, block, 0, st, sa.u, dU, g, ph, gt, sa, \
+                     strips)
+  const bool keep = sa.yOut != nullptr;
+  if (mode == kPlain) { if (w3) SGNX_SWEEP(kPlain, 3, false); else SGNX_SWEEP(kPlain, 2, false); }
+  else if (mode == kAxpy) {
+    if (keep) { if (w3) SGNX_SWEEP(kAxpy, 3, true); else SGNX_SWEEP(kAxpy, 2, true); }
+    else { if (w3) SGNX_SWEEP(kAxpy, 3, false); else SGNX_SWEEP(kAxpy, 2, false); }
+  } else if (mode == kCand) {
+    if (keep) { if (w3) SGNX_SWEEP(kCand, 3, true); else SGNX_SWEEP(kCand, 2, true); }
+    else { if (w3) SGNX_SWEEP(kCand, 3, false); else SGNX_SWEEP(kCand, 2, false); }
+  } else { if (w3) SGNX_SWEEP(kErr, 3, false); else SGNX_SWEEP(kErr, 2, false); }
 #undef SGNX_SWEEP
   return hipGetLastError();
 }
@@ -1099,16 +1111,23 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
 int sweep_strips(const Geometry& g) { return (g.Nx + kTileW - 1) / kTileW; }
 
 hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileStart, int nMem,
-                                  long long totalTiles, int mode, bool waves3, hipStream_t st) {
+                                  long long totalTiles, int mode, bool waves3, hipStream_t st,
+                                  bool keep) {
   if (nMem < 1 || totalTiles < 1 || totalTiles > 0x7fffffffLL) return hipErrorInvalidValue;
   if (mode < kPlain || mode > kErr) return hipErrorInvalidValue;
   const dim3 grid((unsigned)totalTiles), block(Sweep<kTileW>::NT);
 #define SGNX_SWEEPB(M, W) \
   hipLaunchKernelGGL((k_rhs_sweep_batch<kTileW, M, W>), grid, block, 0, st, mem, tileStart, nMem)
+#define SGNX_SWEEPBK(M, W) \
+  hipLaunchKernelGGL((k_rhs_sweep_batch<kTileW, M, W, true>), grid, block, 0, st, mem, tileStart, \
+                     nMem)
   if (mode == kPlain) { if (waves3) SGNX_SWEEPB(kPlain, 3); else SGNX_SWEEPB(kPlain, 2); }
+  else if (mode == kAxpy && keep) { if (waves3) SGNX_SWEEPBK(kAxpy, 3); else SGNX_SWEEPBK(kAxpy, 2); }
   else if (mode == kAxpy) { if (waves3) SGNX_SWEEPB(kAxpy, 3); else SGNX_SWEEPB(kAxpy, 2); }
+  else if (mode == kCand && keep) { if (waves3) SGNX_SWEEPBK(kCand, 3); else SGNX_SWEEPBK(kCand, 2); }
   else if (mode == kCand) { if (waves3) SGNX_SWEEPB(kCand, 3); else SGNX_SWEEPB(kCand, 2); }
   else { if (waves3) SGNX_SWEEPB(kErr, 3); else SGNX_SWEEPB(kErr, 2); }
+#undef SGNX_SWEEPBK
 #undef SGNX_SWEEPB
   return hipGetLastError();
 }

@@ -30,6 +30,10 @@ struct AdjTape {
   std::vector<double*> freeSeg;    // of those, the unused ones
   double* pp[2] = {nullptr, nullptr};   // ping-pong pair for slot-free recomputation
   long recomputed = 0, peakSlots = 0, inUse = 0;   // of the last sweep
+  // stage states kept by the taped run (every step's snapshot held): the
+  // stage-2 and stage-3 inputs of step n, written by its accepted attempt
+  bool keepStages = false, stagesOff = false;
+  std::vector<double*> y2s, y3s;
 };
 
 

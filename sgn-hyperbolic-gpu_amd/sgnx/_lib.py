@@ -121,6 +121,8 @@ SIGNATURES = [
     ("sgnx_tape_begin", C.c_int, [C.c_void_p, C.c_long]),
     ("sgnx_tape_end", C.c_int, [C.c_void_p]),
     ("sgnx_tape_set_slots", C.c_int, [C.c_void_p, C.c_long]),
+    ("sgnx_tape_keep_stages", C.c_int, [C.c_void_p, C.c_int]),
+    ("sgnx_tape_stages", C.c_int, [C.c_void_p, C.POINTER(C.c_long)]),
     ("sgnx_adjoint_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("sgnx_tape_info", C.c_int, [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long),
                                  C.POINTER(C.c_long)]),

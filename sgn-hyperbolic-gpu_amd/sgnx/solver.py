@@ -365,6 +365,19 @@ class SGN2D:
         segment needs; fewer: binomial recomputation, same result)."""
         L.check(self.lib.sgnx_tape_set_slots(self.h, int(slots)), self.h)
 
+    def tapeKeepStages(self, on=True):
+        """Keep each taped step's stage-2 and stage-3 inputs (two more states
+        per step, written by the forward's own stage launches), so the reverse
+        sweep recomputes no stage; dropped once the snapshot budget is full.
+        Set before the run; the adjoint is the same bit for bit."""
+        L.check(self.lib.sgnx_tape_keep_stages(self.h, 1 if on else 0), self.h)
+
+    def tapeStages(self):
+        """Steps whose stage inputs the tape keeps."""
+        a = C.c_long()
+        L.check(self.lib.sgnx_tape_stages(self.h, C.byref(a)), self.h)
+        return a.value
+
     def adjointStats(self):
         """(steps recomputed, slots held at once) of the last adjoint()."""
         a, b = C.c_long(), C.c_long()

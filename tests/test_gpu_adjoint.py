@@ -147,7 +147,7 @@ def test_rhs_vjp_matches_finite_differences(ci):
     s.close()
 
 
-def _taped_run(c, theta, U0, nodes, steps, max_snap):
+def _taped_run(c, theta, U0, nodes, steps, max_snap, keep=False):
     import sgnx
     s = sgnx.SGN2D(**c.solver_kwargs())
     s.setDepth(theta)
@@ -155,6 +155,7 @@ def _taped_run(c, theta, U0, nodes, steps, max_snap):
     s.setGauges(nodes, 4 * steps)
     dt0 = s.cflTimeStep(None, 0.5)
     s.tapeBegin(max_snap)
+    s.tapeKeepStages(keep)
     st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=steps))
     s.tapeEnd()
     assert st.accepted == steps
@@ -250,6 +251,35 @@ def test_checkpoint_budgets_give_the_same_adjoint(budget, slots):
         s.close()
     assert_bitwise(res[1][0], res[0][0], "U0bar under a checkpoint budget")
     assert_bitwise(res[1][1], res[0][1], "thetaBar under a checkpoint budget")
+
+
+@pytest.mark.parametrize("ci", range(4))
+def test_kept_stages_give_the_same_adjoint(ci):
+    """A taped run that keeps its stage inputs (written by the forward's own
+    stage launches, rejected attempts overwritten by the accepted one) runs
+    the same steps to the same state and gauge log, and its reverse sweep,
+    which recomputes no stage, gives the recomputing sweep's cotangents bit
+    for bit; under a snapshot budget the run drops them and still agrees."""
+    c = _cases()[ci]
+    rng = np.random.default_rng(600 + ci)
+    theta = c.depth["Hnod"] + 0.002 * rng.uniform(-1, 1, c.nd)
+    nodes = rng.choice(c.nd, size=5, replace=False).astype(np.int64)
+    steps = 9
+    out = []
+    for bud, keep in ((1 << 20, False), (1 << 20, True), (3, True)):
+        s, UN, G = _taped_run(c, theta, c.U0, nodes, steps, bud, keep)
+        assert s.tapeStages() == (steps if keep and bud > steps else 0)
+        r = np.random.default_rng(1)
+        out.append((UN, G, s.tapeSteps(), s.adjoint(r.normal(size=UN.size), r.normal(size=G.shape))))
+        s.close()
+    (U0, G0, (t0, d0), (ub0, tb0)) = out[0]
+    for tag, (UN, G, (tt, dts), (ub, tb)) in zip(("kept", "dropped"), out[1:]):
+        assert_bitwise(UN, U0, f"{tag}: final state")
+        assert_bitwise(G, G0, f"{tag}: gauge log")
+        assert_bitwise(tt, t0, f"{tag}: step times")
+        assert_bitwise(dts, d0, f"{tag}: step sizes")
+        assert_bitwise(ub, ub0, f"{tag}: U0bar")
+        assert_bitwise(tb, tb0, f"{tag}: thetaBar")
 
 
 def test_inversion_gradient_32_directions():
@@ -428,10 +458,13 @@ def test_batched_adjoint_matches_solo_bitwise():
         s.close()
 
 
-def test_batched_forward_records_the_solo_tapes():
+@pytest.mark.parametrize("keep", [(), (0, 2)])
+def test_batched_forward_records_the_solo_tapes(keep):
     """Members recording tapes inside a batched adaptive run (graph on) get the
     tapes of their solo runs: the same steps, and batched and solo reverse
-    sweeps give the same cotangents bit for bit."""
+    sweeps give the same cotangents bit for bit -- also where some members
+    keep their stage inputs (the batched stage launches write them; the
+    batched sweep recomputes stages for the others only)."""
     import sgnx
     from sgnx import cases
     cs = [cases.gate(Nx=8, Ny=6), cases.solitary(Nx=32, Ny=16), cases.shoal(seed=3, Nx=48, Ny=24)]
@@ -445,10 +478,11 @@ def test_batched_forward_records_the_solo_tapes():
         solo.append((s.tapeSteps(), s.adjoint(ub, gb)))
         cot.append((ub, gb, nodes))
         ss.append(s)
-    for c, s, (_, _, nodes) in zip(cs, ss, cot):
+    for i, (c, s, (_, _, nodes)) in enumerate(zip(cs, ss, cot)):
         s.upload(c.U0)
         s.setGauges(nodes, 64)
         s.tapeBegin(1 << 20)
+        s.tapeKeepStages(i in keep)
     b = sgnx.Batch(ss)
     dt0 = [s.cflTimeStep(None, 0.5) for s in ss]
     for c, s in zip(cs, ss):
@@ -459,6 +493,7 @@ def test_batched_forward_records_the_solo_tapes():
         s.tapeEnd()
     for i, s in enumerate(ss):
         tt, dts = s.tapeSteps()
+        assert s.tapeStages() == (steps[i] if i in keep else 0)
         assert_bitwise(tt, solo[i][0][0], f"member {i} step times")
         assert_bitwise(dts, solo[i][0][1], f"member {i} step sizes")
     res = b.adjoint([u for u, _, _ in cot], [g for _, g, _ in cot])
