@@ -540,6 +540,46 @@ def batched_adjoint_leg(args):
     return out
 
 
+def margin_leg(args):
+    """The large single-grid margin workload (cases.margin: 2000x1000
+    elements, 50 M nodes, 2 GB per state, on 200x100 km; a tanh margin
+    4000 -> 50 m with 2 % roughness, walls in x, periodic in y, an elliptical
+    uplift at rest): --margin-steps accepted adaptive steps in one
+    device-resident call after a short warm-up, the same rate as the main
+    line, plus the plain RHS's rate on the final state."""
+    import sgnx
+    from sgnx import cases
+    t = time.perf_counter()
+    case = cases.margin(seed=args.seed)
+    log(f"[bench] margin case {case.Nx}x{case.Ny} built in {time.perf_counter() - t:.1f}s")
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    s.upload(case.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = dict(atol=case.meta["atol"], rtol=case.meta["rtol"])
+    s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=5, **opts))
+    s.upload(case.U0)
+    m0 = s.mass()
+    t0 = time.perf_counter()
+    st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(max_accepted=args.margin_steps, **opts))
+    wall = time.perf_counter() - t0
+    m1 = s.mass()
+    plain = plain_rhs_rate(s, case.nd)
+    s.close()
+    dof = 5.0 * case.nd
+    rate = dof * st.rhsEvals / wall
+    log(f"[bench] margin: {st.accepted} acc / {st.rejected} rej / {st.rhsEvals} RHS in {wall:.2f}s "
+        f"-> {rate / 1e9:.2f} GDOF-RHS/s, {1e3 * wall / max(st.accepted, 1):.2f} ms per step; "
+        f"relative mass change {abs(m1 - m0) / abs(m0):.1e}")
+    return {"workload": (f"margin {case.Nx}x{case.Ny} P=4 FP64 on 200x100 km: tanh margin "
+                         f"4000 -> 50 m, 2 % roughness, walls in x, periodic in y, uplift at rest "
+                         f"(seed {args.seed}); atol 1e-8 rtol 1e-6"),
+            "nodes": case.nd, "accepted": st.accepted, "rejected": st.rejected,
+            "rhs_evals": st.rhsEvals, "aborted": bool(st.aborted), "wall_s": wall,
+            "dof_rhs_per_s": rate, "ms_per_step": 1e3 * wall / max(st.accepted, 1),
+            "t_reached_s": st.tReached, "mass_rel_change": abs(m1 - m0) / abs(m0),
+            "plain_rhs": plain}
+
+
 def cpu_baseline(case, gpu_solver, dt0, args):
     """The oracle on one core: FSAL head + 2 accepted steps of the same workload."""
     import numpy as np
@@ -598,6 +638,8 @@ def main():
     ap.add_argument("--adjoint-members", type=int, default=16,
                     help="members of the batched adjoint leg (0 = skip)")
     ap.add_argument("--adjoint-member-steps", type=int, default=60)
+    ap.add_argument("--margin-steps", type=int, default=1000,
+                    help="accepted steps of the 2000x1000 margin leg (0 = skip)")
     ap.add_argument("--profile-rhs", type=int, default=0)
     ap.add_argument("--profile-vjp", type=int, default=0,
                     help="profile mode: this many RHS vector-Jacobian products")
@@ -709,6 +751,14 @@ def main():
             batched_adjoint = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] batched adjoint leg failed: {batched_adjoint['error']}")
 
+    margin = None
+    if args.margin_steps > 0:
+        try:
+            margin = margin_leg(args)
+        except Exception as e:                    # reported, not fatal to the main line
+            margin = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] margin leg failed: {margin['error']}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -755,6 +805,7 @@ def main():
         "adjoint": adjoint,
         "inversion": inversion,
         "batched_adjoint": batched_adjoint,
+        "margin": margin,
     }
     print(json.dumps(out), flush=True)
 

@@ -19,6 +19,10 @@ Workloads
   inversion  256x128 on 64x32 m, 0.5 m minus six seeded mounds, walls, eight
              seeded surface humps at rest, 20 seeded gauges (the adjoint's
              gauge-inversion workload)
+  margin     2000x1000 on 200x100 km: a tanh continental margin 4000 -> 50 m
+             across a seeded shelf break with 2 % seeded roughness, walls in x,
+             periodic in y, a seeded 0.5-2 m elliptical Gaussian uplift at rest
+             (the large single-grid case)
 """
 from __future__ import annotations
 
@@ -319,6 +323,54 @@ def inversion(seed=0, Nx=256, Ny=128, P=4, n_gauges=20):
     return Case("inversion", P, Nx, Ny, Lx, Ly, par, depth, ("wall",) * 4, U0,
                 meta={"atol": 1e-10, "rtol": 1e-8, "cfl": 0.5, "steps": 2000, "seed": seed,
                       "theta": theta, "eta0": eta0, "gauges": gauges, "checkpoints": 40})
+
+
+def margin_params(seed):
+    rng = np.random.default_rng(seed)
+    return {
+        "x_break": float(rng.uniform(0.55, 0.75)),   # shelf break, fraction of Lx
+        "width": float(rng.uniform(5.0e3, 15.0e3)),  # slope half-width [m]
+        "amp": float(rng.uniform(0.5, 2.0)),         # uplift [m]
+        "xs": float(rng.uniform(0.2, 0.4)),          # uplift centre, fractions of Lx, Ly
+        "ys": float(rng.uniform(0.3, 0.7)),
+        "sx": float(rng.uniform(5.0e3, 2.0e4)),      # uplift semi-axes [m]
+        "sy": float(rng.uniform(1.0e4, 4.0e4)),
+    }
+
+
+def margin(seed=0, Nx=2000, Ny=1000, P=4, rough=0.02):
+    """The large single-grid margin workload, a regional cut of the
+    co-seismic uplift over a continental margin of
+    apps/tsunami_margin_mpi.cu:130-172: 200x100 km, still depth
+    50 + 3950 * (1 - tanh((x - x_break) / width)) / 2 (abyssal plain 4000 m to
+    the west, shelf 50 m to the east) times (1 + rough * seeded per-point noise),
+    walls on the two x sides, periodic along the margin (y), the fluid at rest
+    under a seeded elliptical Gaussian uplift over the deep basin;
+    c = 3 sqrt(g 4000) from the greatest depth, as there."""
+    Lx, Ly = 2.0e5, 1.0e5
+    h_deep, h_shelf = 4000.0, 50.0
+    par = Params(g=G, c=ALPHA * math.sqrt(G * h_deep), gamma=GAMMA)
+    mp = margin_params(seed)
+    xb = mp["x_break"] * Lx
+
+    def depth_fn(x, y):
+        s = 0.5 * (1.0 - np.tanh((x - xb) / mp["width"]))
+        H = h_shelf + (h_deep - h_shelf) * s
+        # the roughness is periodic in y like the domain: y = Ly is y = 0
+        yw = y - Ly * np.floor(y / Ly + 1e-9)
+        return H * (1.0 + rough * point_noise(x, yw, seed))
+
+    bc = ("wall", "wall", "periodic", "periodic")
+    depth = depth_arrays(P, Nx, Ny, Lx, Ly, depth_fn, bc)
+    x, y = node_coords(P, Nx, Ny, Lx, Ly)
+    dx = (x - mp["xs"] * Lx) / mp["sx"]
+    dy = (y - mp["ys"] * Ly) / mp["sy"]
+    Hn = depth["Hnod"]
+    z = np.zeros_like(Hn)
+    U0 = prim_to_state(Hn + mp["amp"] * np.exp(-(dx * dx + dy * dy)), z, z, z, z)
+    return Case("margin", P, Nx, Ny, Lx, Ly, par, depth, bc, U0,
+                meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "steps": 1000, "seed": seed,
+                      "margin": mp, "rough": rough})
 
 
 def smooth_direction(case, rng, n=4, amp=0.01, near=None):

@@ -1,0 +1,103 @@
+"""The large single-grid margin workload (cases.margin: 2000x1000 elements on
+200x100 km, a tanh continental margin 4000 -> 50 m with 2 % seeded roughness,
+walls in x, periodic in y, an elliptical uplift at rest) against the oracle:
+the RHS bit for bit at reduced and at full size, an adaptive trajectory at
+reduced size to the trajectory bar (same step counts, relative L2 <= 1e-12),
+and mass conservation of an adaptive run at full size."""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from test_gpu_rhs import assert_bitwise
+from test_gpu_rk import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("Nx,Ny", [(40, 20), (37, 11)])
+def test_margin_rhs_bitwise(Nx, Ny):
+    """The initial state and a moving state (the uplift plus seeded currents),
+    both kernel variants."""
+    import sgnx
+    from sgnx import cases
+    c = cases.margin(seed=1, Nx=Nx, Ny=Ny)
+    rng = np.random.default_rng(3)
+    U = c.U0.copy()
+    h = U[:c.nd]
+    for v, amp in ((1, 0.3), (2, 0.2), (3, 0.05), (4, 0.5)):
+        U[v * c.nd:(v + 1) * c.nd] = h * amp * rng.uniform(-1, 1, c.nd)
+    o = OL.Oracle.from_case(c)
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    for tag, X in (("initial", c.U0), ("moving", U)):
+        want = o.rhs(X)
+        for variant, rows in ((1, 16), (1, 3), (0, 16)):
+            s.set_rhs_kernel(variant, rows)
+            assert_bitwise(s.rhs(X), want, f"margin {Nx}x{Ny} {tag} variant {variant}/{rows}")
+    s.close()
+    o.close()
+
+
+def test_margin_adaptive_matches_oracle():
+    """40 accepted adaptive steps at 80x40 (atol 1e-8, rtol 1e-6): the same
+    accepted and rejected counts as the oracle's run and a relative L2
+    distance <= 1e-12."""
+    import sgnx
+    from sgnx import cases
+    c = cases.margin(seed=2, Nx=80, Ny=40)
+    o = OL.Oracle.from_case(c)
+    dt0 = o.cfl_time_step(c.U0, 0.5)
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.upload(c.U0)
+    st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=40))
+    Ug = s.download()
+    s.close()
+    Uo, sto, _, _, _ = o.integrate(c.U0, 0.0, 1e9, dt0,
+                                   OL.OrcOpts(atol=1e-8, rtol=1e-6, max_accepted=40))
+    o.close()
+    assert (st.accepted, st.rejected, st.rhsEvals) == (sto.accepted, sto.rejected, sto.rhsEvals)
+    assert st.accepted == 40
+    assert rel_l2(Ug, Uo) <= 1e-12
+
+
+@pytest.fixture(scope="module")
+def margin_full():
+    from sgnx import cases
+    return cases.margin(seed=0)
+
+
+def test_margin_fullsize_rhs_bitwise(margin_full):
+    """2000x1000 (50 M nodes, 2 GB per state): one RHS of the initial state
+    against the oracle, bit for bit."""
+    import sgnx
+    c = margin_full
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    got = s.rhs(c.U0)
+    s.close()
+    o = OL.Oracle.from_case(c)
+    want = o.rhs(c.U0)
+    o.close()
+    assert_bitwise(got, want, "margin 2000x1000")
+
+
+def test_margin_fullsize_mass_conserved(margin_full):
+    """Walls in x and a periodic y axis conserve mass: 5 adaptive steps at
+    full size change it by round-off only; the run is finite and
+    reproducible bit for bit."""
+    import sgnx
+    c = margin_full
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.upload(c.U0)
+    m0 = s.mass()
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=5)
+    st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    m1 = s.mass()
+    U1 = s.download()
+    s.upload(c.U0)
+    st2 = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    U2 = s.download()
+    s.close()
+    assert st.accepted == 5 and st.as_dict() == st2.as_dict()
+    assert np.all(np.isfinite(U1))
+    assert abs(m1 - m0) <= 1e-12 * abs(m0), (m0, m1)
+    assert_bitwise(U2, U1, "margin rerun")

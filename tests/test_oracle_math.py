@@ -183,6 +183,26 @@ def test_mass_conserved_with_walls():
         assert abs(o.mass(U) - M0) / abs(M0) < 1e-13
 
 
+def test_margin_lake_at_rest_and_mass():
+    """The margin workload at 24x12 on the oracle: over its rough 4000 -> 50 m
+    margin the fluid at rest stays at rest (RHS zero to round-off relative to
+    g H), and the uplift's adaptive run conserves mass with walls in x and a
+    periodic y axis."""
+    from sgnx import cases
+    c = cases.margin(seed=5, Nx=24, Ny=12)
+    o = OL.Oracle.from_case(c)
+    Hn = c.depth["Hnod"]
+    z = np.zeros_like(Hn)
+    R = o.rhs(cases.prim_to_state(Hn.copy(), z, z, z, z))
+    assert np.abs(R).max() < 1e-12 * 9.81 * Hn.max() ** 2
+    M0 = o.mass(c.U0)
+    dt0 = o.cfl_time_step(c.U0, 0.5)
+    U, st, _, _, _ = o.integrate(c.U0, 0.0, 1e9, dt0, OL.OrcOpts(max_accepted=20))
+    assert st.accepted == 20 and np.all(np.isfinite(U))
+    assert abs(o.mass(U) - M0) / abs(M0) < 1e-13
+    o.close()
+
+
 def test_inflow_ghost_is_a_linear_eigenmode():
     """Rows 1-3 of the linearised system fix the ghost amplitudes; row 4
     (hp equation) then holds only if k is on the slow branch (test/
