@@ -375,7 +375,7 @@ bool tape_stage_outputs(const sgnx_solver* s, double** y2, double** y3) {
   return true;
 }
 
-int tape_step_begin(sgnx_solver* s, hipStream_t st) {
+int tape_step_begin(sgnx_solver* s) {
   AdjTape* tp = s->tape;
   const long n = (long)tp->t.size();
   int rc;
@@ -401,18 +401,26 @@ int tape_step_begin(sgnx_solver* s, hipStream_t st) {
     tp->K *= 2;
     if (n % tp->K != 0) return SGNX_OK;
   }
-  double* p = nullptr;
-  if ((rc = take_spare(s, tp, &p))) return rc;
-  tp->snap.push_back(p);
-  HIPCHK(s, hipMemcpyAsync(p, s->d_U, s->n() * sizeof(double), hipMemcpyDeviceToDevice, st));
+  tp->pendingSnap = true;
   return SGNX_OK;
 }
 
-void tape_accept(sgnx_solver* s, double t, double dt, long row) {
+int tape_accept(sgnx_solver* s, double t, double dt, long row) {
   AdjTape* tp = s->tape;
   tp->t.push_back(t);
   tp->dt.push_back(dt);
   tp->row.push_back(row);
+  if (tp->pendingSnap) {
+    // no copy: the buffer holding the accepted step's input becomes the
+    // snapshot, and the solver takes a spare one for its next candidate
+    double* fresh = nullptr;
+    const int rc = take_spare(s, tp, &fresh);
+    if (rc) return rc;
+    tp->snap.push_back(s->d_uNew);
+    s->d_uNew = fresh;
+    tp->pendingSnap = false;
+  }
+  return SGNX_OK;
 }
 
 void tape_free(sgnx_solver* s) {
@@ -491,6 +499,7 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots) {
   release_snapshots(tp);
   release_stages(tp);
   tp->stagesOff = false;
+  tp->pendingSnap = false;
   // buffers beyond the new budget (three states a step while stages are
   // kept) are returned to the device
   const long cap = tp->keepStages ? 3 * max_snapshots : max_snapshots;
@@ -541,7 +550,10 @@ int sgnx_adjoint_stats(const sgnx_solver* s, long* recomputed_steps, long* peak_
 
 int sgnx_tape_end(sgnx_solver* s) {
   if (!s) return SGNX_EINVAL;
-  if (s->tape) s->tape->recording = false;
+  if (s->tape) {
+    s->tape->recording = false;
+    s->tape->pendingSnap = false;
+  }
   return SGNX_OK;
 }
 

@@ -318,7 +318,7 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
   while (c.next()) {
     const double t = c.t, dt = c.dt;
     if (taping && stepBegun != c.S.accepted) {
-      if ((rc = tape_step_begin(s, s->stream))) return rc;
+      if ((rc = tape_step_begin(s))) return rc;
       stepBegun = c.S.accepted;
     }
     if (s->g.variant == 1) {
@@ -345,7 +345,7 @@ int sgnx_integrate(sgnx_solver* s, double t0, double tEnd, double dt0, const sgn
       std::swap(s->d_k1, s->d_k4);
       const long rows = s->grows;
       HIPCHK(s, record_gauges(s, s->stream, c.t));
-      if (taping) tape_accept(s, t, dt, s->grows > rows ? rows : -1);
+      if (taping && (rc = tape_accept(s, t, dt, s->grows > rows ? rows : -1))) return rc;
     }
     if (c.done) break;
   }

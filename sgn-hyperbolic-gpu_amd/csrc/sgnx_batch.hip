@@ -281,12 +281,11 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
       if (ctl[i].next()) set.push_back(i);
     if (set.empty()) break;
     const int A = (int)set.size();
-    // members recording a tape: the step's input state, on the batch stream
-    // ahead of the step (outside any captured graph), and the buffers its
-    // stage launches keep their inputs in
+    // members recording a tape: whether the step's input state is due as a
+    // snapshot, and the buffers its stage launches keep their inputs in
     for (int slot = 0; slot < A; ++slot) {
       sgnx_solver* s = b->m[set[slot]];
-      if (tape_recording(s) && tape_step_begin(s, b->stream))
+      if (tape_recording(s) && tape_step_begin(s))
         return bfail(b, SGNX_EDEVICE, "batch: tape snapshot: " + s->err);
     }
     bool keep = false;
@@ -332,7 +331,8 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
         std::swap(s->d_k1, s->d_k4);
         const long rows = s->grows;
         BCHK(b, record_gauges(s, b->stream, ctl[i].t));
-        if (tape_recording(s)) tape_accept(s, tStep, dtStep, s->grows > rows ? rows : -1);
+        if (tape_recording(s) && tape_accept(s, tStep, dtStep, s->grows > rows ? rows : -1))
+          return bfail(b, SGNX_EDEVICE, "batch: tape: " + s->err);
       }
     }
   }

@@ -55,16 +55,18 @@ struct sgnx_solver {
 };
 
 // Adjoint tape hooks of the adaptive loops (sgnx_adjoint_api.hip): before the
-// first attempt of every step (snapshot of the step's input state when due,
-// copied on stream st),
-// after every accepted step (its time, step size and gauge log row), and the
-// release of the tape's buffers.
+// first attempt of every step (whether the step's input state is to be a
+// snapshot; buffers for kept stage inputs), after every accepted step, once
+// d_U and d_uNew are swapped (its time, step size and gauge log row; a due
+// snapshot takes over d_uNew, which then holds the step's input, and the
+// solver gets a spare buffer in its place), and the release of the tape's
+// buffers.
 bool tape_recording(const sgnx_solver* s);
 // While a tape keeps stage states: the buffers the current step's stage 2 and
 // stage 3 launches write their stage inputs to (false: none).
 bool tape_stage_outputs(const sgnx_solver* s, double** y2, double** y3);
-int tape_step_begin(sgnx_solver* s, hipStream_t st);
-void tape_accept(sgnx_solver* s, double t, double dt, long row);
+int tape_step_begin(sgnx_solver* s);
+int tape_accept(sgnx_solver* s, double t, double dt, long row);
 void tape_free(sgnx_solver* s);
 
 // eta at the gauges of s into its next log row (after an accepted step at t)

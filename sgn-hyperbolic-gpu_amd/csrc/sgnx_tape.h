@@ -17,6 +17,7 @@ struct AdjTape {
   long maxSnap = 0;
   long K = 1;                      // snapshot stride in steps
   std::vector<double*> snap;       // snap[j] = input state of step j*K
+  bool pendingSnap = false;        // the step under way is to be snapshot on acceptance
   std::vector<double*> spare;      // allocated, unused snapshot buffers
   std::vector<double> t, dt;       // accepted steps
   std::vector<long> row;           // gauge log row written after step n, or -1

@@ -282,6 +282,41 @@ def test_kept_stages_give_the_same_adjoint(ci):
         assert_bitwise(tb, tb0, f"{tag}: thetaBar")
 
 
+@pytest.mark.parametrize("budget,keep", [(1 << 20, False), (3, False), (1 << 20, True)])
+def test_taped_run_leaves_the_solver_intact(budget, keep):
+    """Snapshots take over the buffer of each accepted step's input instead of
+    copying it: a taped run, its reverse sweep and a further untaped run give
+    the states, step counts and gauge logs of the same calls on an untaped
+    solver, bit for bit."""
+    import sgnx
+    c = _cases()[3]
+    nodes = np.array([3, 140, 611], dtype=np.int64)
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=9)
+    out = []
+    for taped in (True, False):
+        s = sgnx.SGN2D(**c.solver_kwargs())
+        s.setDepth(c.depth["Hnod"])
+        s.upload(c.U0)
+        s.setGauges(nodes, 64)
+        dt0 = s.cflTimeStep(None, 0.5)
+        if taped:
+            s.tapeKeepStages(keep)
+            s.tapeBegin(budget)
+        st1 = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+        U1 = s.download()
+        if taped:
+            s.tapeEnd()
+            s.adjoint(np.ones(U1.size), None)
+        st2 = s.integrateAdaptive(st1.tReached, 1e9, st1.dtLast, opts)
+        out.append((st1.as_dict(), st2.as_dict(), U1, s.download(), s.gaugeLog()[1]))
+        s.close()
+    a, b = out
+    assert a[0] == b[0] and a[1] == b[1]
+    for k, what in ((2, "state after the taped run"), (3, "state after the next run"),
+                    (4, "gauge log")):
+        assert_bitwise(a[k], b[k], what)
+
+
 def test_inversion_gradient_32_directions():
     """The gauge-inversion workload (smaller grid and run): the gradient of
     the gauge misfit 0.5 |G(eta0, theta) - G_obs|^2 with respect to the initial
