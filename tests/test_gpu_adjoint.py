@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as OL
+from tlm_lib import tangent_linear
 from test_gpu_rhs import assert_bitwise
 
 pytestmark = pytest.mark.gpu
@@ -315,6 +316,51 @@ def test_taped_run_leaves_the_solver_intact(budget, keep):
     for k, what in ((2, "state after the taped run"), (3, "state after the next run"),
                     (4, "gauge log")):
         assert_bitwise(a[k], b[k], what)
+
+
+def _plain_rhs(s):
+    def rhs(U, th, t):
+        s.setDepth(th)
+        return s.rhs(U, t)
+    return rhs
+
+
+@pytest.mark.parametrize("ci", range(5))
+def test_adjoint_matches_tangent_linear_model(ci):
+    """The adjoint's directional derivatives <U0bar, dU0> + <thetaBar, dtheta>
+    against the tangent-linear reference computed with the plain solver's RHS,
+    in three seeded random (rough) directions of the initial state and the
+    still depth: the four small cases over 6 adaptive steps, and the
+    gauge-inversion workload at 48x24 over 60 steps with its 20 gauges."""
+    import sgnx
+    from sgnx import cases
+    if ci < 4:
+        c = _cases()[ci]
+        steps = 6
+        rng = np.random.default_rng(800 + ci)
+        theta = c.depth["Hnod"] + 0.002 * rng.uniform(-1, 1, c.nd)
+        U0 = c.U0 + 1e-3 * rng.uniform(-1, 1, c.U0.size) * np.repeat([1, 0.5, 0.5, 0.2, 0.2], c.nd)
+        nodes = rng.choice(c.nd, size=6, replace=False).astype(np.int64)
+    else:
+        c = cases.inversion(seed=6, Nx=48, Ny=24)
+        steps = 60
+        rng = np.random.default_rng(806)
+        theta, U0, nodes = c.meta["theta"], c.U0, c.meta["gauges"]
+    s, UN, G = _taped_run(c, theta, U0, nodes, steps, 1 << 20, keep=True)
+    Ubar = rng.normal(size=UN.size)
+    Gbar = rng.normal(size=G.shape)
+    U0b, thb = s.adjoint(Ubar, Gbar)
+    tt, dts = s.tapeSteps()
+    rhs = _plain_rhs(s)
+    for trial in range(3):
+        dU = rng.normal(size=U0.size)
+        dth = rng.normal(size=c.nd)
+        ad = U0b @ dU + thb @ dth
+        tl = tangent_linear(rhs, U0, theta, dU, dth, list(zip(tt, dts)), nodes, Ubar, Gbar)
+        err = _rel(ad, tl)
+        print(f"{c.name} direction {trial}: adjoint {ad:.12e} tangent-linear {tl:.12e} rel {err:.1e}")
+        assert err <= TOL, (ci, trial, ad, tl)
+    s.close()
 
 
 def test_inversion_gradient_32_directions():

@@ -203,6 +203,59 @@ def test_margin_lake_at_rest_and_mass():
     o.close()
 
 
+def test_tangent_linear_reference_against_run_differences():
+    """The tangent-linear reference of the adjoint tests (tlm_lib), driven by
+    the oracle's RHS, against Richardson-extrapolated central differences of
+    the oracle's whole 6-step BS3 run (2 C(5e-4) - C(1e-3)) along three
+    smooth directions of the state and the still depth: within 2e-5
+    (the run differences' own error)."""
+    from sgnx import cases
+    from tlm_lib import tangent_linear
+    from test_gpu_adjoint import theta_depth
+    c = cases.gate(Nx=8, Ny=6)
+    rng = np.random.default_rng(11)
+    theta = c.depth["Hnod"].copy()
+
+    def orc(th):
+        return OL.Oracle(c.P, c.Nx, c.Ny, c.Lx, c.Ly, c.par, theta_depth(th, c.Nx, c.Ny, c.bc),
+                         c.bc, c.inflow)
+
+    def rhs(U, th, t):
+        o = orc(th)
+        r = o.rhs(U, t)
+        o.close()
+        return r
+
+    o0 = orc(theta)
+    dt0 = o0.cfl_time_step(c.U0, 0.5)
+    o0.close()
+    steps = [(i * dt0, dt0) for i in range(6)]
+    nodes = rng.choice(c.nd, 5, replace=False)
+    Ubar = rng.normal(size=c.U0.size)
+    Gbar = rng.normal(size=(6, 5))
+
+    def loss(U0, th):
+        o = orc(th)
+        U = U0.copy()
+        L = 0.0
+        for k, (t, dt) in enumerate(steps):
+            U = o.bs3_step(U, t, dt)
+            L += Gbar[k] @ (U[nodes] - th[nodes])
+        o.close()
+        return L + Ubar @ U
+
+    for _ in range(3):
+        dth = cases.smooth_direction(c, rng, n=3, amp=1.0)
+        dU = np.concatenate([cases.smooth_direction(c, rng, n=3, amp=1.0) for _ in range(5)])
+        tl = tangent_linear(rhs, c.U0, theta, dU, dth, steps, nodes, Ubar, Gbar)
+
+        def central(eps):
+            return (loss(c.U0 + eps * dU, theta + eps * dth) -
+                    loss(c.U0 - eps * dU, theta - eps * dth)) / (2.0 * eps)
+        fd = 2.0 * central(5e-4) - central(1e-3)
+        assert abs(tl - fd) <= 2e-5 * abs(tl), (tl, fd)
+
+
 def test_inflow_ghost_is_a_linear_eigenmode():
     """Rows 1-3 of the linearised system fix the ghost amplitudes; row 4
     (hp equation) then holds only if k is on the slow branch (test/
