@@ -15,13 +15,15 @@ integration of exactly --steps accepted steps, timed between two device
 synchronisations (state resident in HBM before the clock starts).
 
 Also reported:
-  roofline      the RHS kernel (k_rhs_sweep): its algorithmic bytes per launch
-                (104 B per node: U read, dU written, H, Hx, Hy read) over the
-                mean duration of the timed run's RHS launches, measured with HIP
-                events around every launch, against the 8 TB/s HBM3E peak.
-                Beside it: the same launches counting the RK stage operands each
-                fused launch kind also reads and writes (ALG_BYTES_BY_KIND), and
-                the plain RHS on its own (20 launches after the timed run);
+  roofline      the RHS kernel (k_rhs_sweep) as the timed run launches it, with
+                the RK stage fused: the algorithmic bytes of each launch kind
+                (ALG_BYTES_BY_KIND: the RHS's 104 B per node -- U read, dU
+                written, H, Hx, Hy read -- plus the stage operands that launch
+                reads and writes), averaged over the run's launches, over their
+                mean duration, measured with HIP events around every launch,
+                against the 8 TB/s HBM3E peak. Beside it: the same launches
+                priced at the RHS's 104 B per node only, and the plain RHS on
+                its own (20 launches after the timed run);
                 `traffic` is the
                 measured HBM bytes per launch from the rocprofv3 counter runs
                 summarised in profiles/rhs_traffic.json (per launch kind,
@@ -695,8 +697,9 @@ def main():
     # duration; and, beside it, the bytes each fused launch kind also moves for
     # its RK stage operands
     achieved = ALG_BYTES_PER_NODE * nd / mean_rhs_s / 1e9
-    achieved_stage = sum(ALG_BYTES_BY_KIND[m] * nd * timing[m][0]
-                         for m in range(len(timing))) / (rhs_ms / 1e3) / 1e9
+    alg_stage_per_launch = sum(ALG_BYTES_BY_KIND[m] * nd * timing[m][0]
+                               for m in range(len(timing))) / launches
+    achieved_stage = alg_stage_per_launch / mean_rhs_s / 1e9
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE runs,
     # copy-calibrated, per launch kind), weighted by this run's launch mix
     traffic = None
@@ -783,17 +786,18 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": achieved_stage,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": achieved_stage / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_rhs_sweep (FR/SGN right-hand side, stage input fused)",
-            "alg_bytes_per_launch": ALG_BYTES_PER_NODE * nd,
+            "kernel": "k_rhs_sweep (FR/SGN right-hand side with the RK stage fused)",
+            "alg_bytes_per_launch": alg_stage_per_launch,
+            "alg_bytes_per_node_mean": alg_stage_per_launch / nd,
             "mean_launch_ms": mean_rhs_s * 1e3,
             "launches": launches,
-            "achieved_incl_stage_operands": achieved_stage,
-            "frac_incl_stage_operands": achieved_stage / HBM_PEAK_GBS,
+            "achieved_rhs_bytes_only": achieved,
+            "frac_rhs_bytes_only": achieved / HBM_PEAK_GBS,
             "plain_rhs": plain,
             "launches_by_kind": {k: timing[m][0] for m, k in enumerate(KIND_NAMES)},
             "mean_ms_by_kind": {k: (timing[m][1] / timing[m][0] if timing[m][0] else None)
