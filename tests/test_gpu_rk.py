@@ -108,6 +108,51 @@ def test_adaptive_solitary_200_steps():
     assert st.tReached == pytest.approx(sto.tReached, rel=1e-12)
 
 
+def test_solitary_first_10_steps_rhs_bitwise():
+    """The solitary-wave workload's gate: every RHS evaluation of the oracle's
+    first 10 accepted adaptive steps (the FSAL head and three per attempt:
+    y2, y3 and the candidate; 31 without rejections), evaluated by the GPU on
+    the same inputs, bit for bit, with the row sweep and the tile kernel; the
+    outputs' digests and times are those the oracle's own run logged, in
+    order, so the inputs are its stage states."""
+    import sgnx
+    from sgnx import cases
+    case = cases.solitary()
+    o = OL.Oracle.from_case(case)
+    dt0 = o.cfl_time_step(case.U0, 0.5)
+    _, sto, olog, ct, ch = o.integrate(case.U0, 0.0, 1e9, dt0,
+                                       OL.OrcOpts(atol=1e-8, rtol=1e-6, max_accepted=10),
+                                       step_cap=64, call_cap=64)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    u, t = case.U0.copy(), 0.0
+    evals = 0
+
+    def both(X, tt):
+        nonlocal evals
+        want = o.rhs(X, tt)
+        for variant, rows in ((1, 16), (0, 16)):
+            s.set_rhs_kernel(variant, rows)
+            got = s.rhs(X, tt)
+            assert_bitwise(got, want, f"RHS evaluation {evals}, variant {variant}")
+        assert OL.fnv64(got) == int(ch[evals]) and tt == ct[evals], f"RHS evaluation {evals}"
+        evals += 1
+        return want
+
+    k1 = both(u, t)
+    for i in range(sto.accepted + sto.rejected):
+        dt, err = olog[i]
+        # rk.cpp:99-108
+        k2 = both(u + 0.5 * dt * k1, t + 0.5 * dt)
+        k3 = both(u + 0.75 * dt * k2, t + 0.75 * dt)
+        un = u + dt * ((2.0 / 9.0 * k1 + 1.0 / 3.0 * k2) + 4.0 / 9.0 * k3)
+        k4 = both(un, t + dt)
+        if err <= 1.0:
+            u, t, k1 = un, t + dt, k4
+    s.close()
+    o.close()
+    assert sto.accepted == 10 and evals == 1 + 3 * (sto.accepted + sto.rejected)
+
+
 def test_adaptive_abort_on_dtmin():
     """dtMin above the step the controller wants -> aborted, like rk.cpp:155-160."""
     from sgnx import cases
