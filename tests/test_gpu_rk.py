@@ -108,16 +108,21 @@ def test_adaptive_solitary_200_steps():
     assert st.tReached == pytest.approx(sto.tReached, rel=1e-12)
 
 
-def test_solitary_first_10_steps_rhs_bitwise():
+@pytest.mark.parametrize("walls", [True, False])
+def test_solitary_first_10_steps_rhs_bitwise(walls):
     """The solitary-wave workload's gate: every RHS evaluation of the oracle's
     first 10 accepted adaptive steps (the FSAL head and three per attempt:
     y2, y3 and the candidate; 31 without rejections), evaluated by the GPU on
     the same inputs, bit for bit, with the row sweep and the tile kernel; the
     outputs' digests and times are those the oracle's own run logged, in
-    order, so the inputs are its stage states."""
+    order, so the inputs are its stage states. On the all-periodic grid the
+    reference's own run is in the fixtures: the GPU's first 31 outputs carry
+    its digests (tests/golden/golden.json, "first10")."""
+    import json
+    import os
     import sgnx
     from sgnx import cases
-    case = cases.solitary()
+    case = cases.solitary(walls=walls)
     o = OL.Oracle.from_case(case)
     dt0 = o.cfl_time_step(case.U0, 0.5)
     _, sto, olog, ct, ch = o.integrate(case.U0, 0.0, 1e9, dt0,
@@ -126,6 +131,7 @@ def test_solitary_first_10_steps_rhs_bitwise():
     s = sgnx.SGN2D(**case.solver_kwargs())
     u, t = case.U0.copy(), 0.0
     evals = 0
+    gpu_digests = []
 
     def both(X, tt):
         nonlocal evals
@@ -134,7 +140,8 @@ def test_solitary_first_10_steps_rhs_bitwise():
             s.set_rhs_kernel(variant, rows)
             got = s.rhs(X, tt)
             assert_bitwise(got, want, f"RHS evaluation {evals}, variant {variant}")
-        assert OL.fnv64(got) == int(ch[evals]) and tt == ct[evals], f"RHS evaluation {evals}"
+        gpu_digests.append(OL.fnv64(got))
+        assert gpu_digests[-1] == int(ch[evals]) and tt == ct[evals], f"RHS evaluation {evals}"
         evals += 1
         return want
 
@@ -151,6 +158,15 @@ def test_solitary_first_10_steps_rhs_bitwise():
     s.close()
     o.close()
     assert sto.accepted == 10 and evals == 1 + 3 * (sto.accepted + sto.rejected)
+    if not walls:
+        path = os.path.join(os.path.dirname(__file__), "golden", "golden.json")
+        with open(path) as f:
+            m = json.load(f)["solitary_128x64_periodic"]
+        if OL.fnv64_py(case.U0) != m["IC_digest"]:
+            pytest.skip("initial state differs from the fixture's (host libm differs)")
+        want = [int(v) for v in m["first10"]["call_h"]]
+        assert [int(h) for h in ch[:len(want)]] == want
+        assert gpu_digests[:len(want)] == want
 
 
 def test_adaptive_abort_on_dtmin():
