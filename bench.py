@@ -608,6 +608,7 @@ def cpu_baseline(case, gpu_solver, dt0, args):
     log(f"[bench] cpu: {sto.rhsEvals} RHS in {t_int:.2f}s ({value / 1e6:.1f} MDOF-RHS/s), "
         f"single RHS {t_rhs:.2f}s; full-size RHS bitwise={rhs_bitwise}, "
         f"2-step rel L2={rel:.2e}")
+    agg = cpu_aggregate(case, OL)
     return {"value": value, "unit": "DOF-RHS-evals/s", "cores": 1, "kind": "port",
             "sample": (f"CPU oracle (reference path restated op for op, g++ -O2 "
                        f"-ffp-contract=off, 1 thread) on the same {case.Nx}x{case.Ny} workload: "
@@ -615,7 +616,40 @@ def cpu_baseline(case, gpu_solver, dt0, args):
                        f"in {t_int:.2f} s"),
             "check": {"full_size_rhs_bitwise": rhs_bitwise, "steps2_rel_l2": rel,
                       "steps2_counts_equal": (stg.accepted, stg.rejected, stg.rhsEvals)
-                      == (sto.accepted, sto.rejected, sto.rhsEvals)}}
+                      == (sto.accepted, sto.rejected, sto.rhsEvals)},
+            "aggregate": agg}
+
+
+def cpu_aggregate(case, OL, reps=2):
+    """N independent single-threaded oracle instances at once (N = the host
+    cores this process may use, at most 16), each evaluating the full-size RHS
+    `reps` times: the host's aggregate rate. ctypes releases the interpreter
+    lock during the oracle's calls, so the threads run in parallel."""
+    import threading
+    n = min(16, len(os.sched_getaffinity(0)))
+    orcs = [OL.Oracle.from_case(case) for _ in range(n)]
+    start = threading.Barrier(n + 1)
+
+    def work(o):
+        start.wait()
+        for _ in range(reps):
+            o.rhs(case.U0)
+
+    th = [threading.Thread(target=work, args=(o,)) for o in orcs]
+    for x in th:
+        x.start()
+    start.wait()
+    t = time.perf_counter()
+    for x in th:
+        x.join()
+    wall = time.perf_counter() - t
+    for o in orcs:
+        o.close()
+    value = 5.0 * case.nd * n * reps / wall
+    log(f"[bench] cpu aggregate: {n} instances x {reps} RHS in {wall:.2f}s "
+        f"({value / 1e6:.1f} MDOF-RHS/s)")
+    return {"value": value, "unit": "DOF-RHS-evals/s", "cores": n, "instances": n,
+            "sample": f"{n} concurrent single-threaded oracle instances, {reps} full-size RHS each"}
 
 
 def main():
