@@ -31,11 +31,23 @@ Also reported:
   cpu_baseline  the CPU oracle (the reference path restated op for op) on one
                 host core over a bounded sample of the same workload: the FSAL
                 head plus two accepted steps; the same sample also checks the
-                full-size RHS bitwise and the 2-step trajectory.
+                full-size RHS bitwise and the 2-step trajectory. Beside it, the
+                aggregate of N concurrent single-threaded instances.
+  batched       the 64-member uneven sweep (Nx 128..768): one at a time against
+                one batch (one launch per stage kind, captured step graph),
+                members bit-identical
+  adjoint       a gradient at 1024x512 (taped run + reverse sweep) against the
+                plain run, with and without kept stage inputs
+  inversion     the gauge-inversion gradient (256x128, 2000 steps, 16 snapshots +
+                20 recomputation slots) checked by differences in 32 directions
+  batched_adjoint  16 uneven members' gradients, one by one against one batch,
+                each checked by differences
+  margin        the 2000x1000 margin workload (50 M nodes), 1000 adaptive steps
 
 usage: python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--seed 0]
        python bench.py --profile-rhs N    (N RHS launches + a copy of known size,
                                            for rocprofv3 counter runs)
+       python bench.py --profile-vjp N    (N RHS vector-Jacobian products)
 """
 from __future__ import annotations
 
