@@ -442,7 +442,7 @@ def batched_adjoint_leg(args):
     (plain run, taped run, reverse sweep) and as one batch (a batched taped
     run, one launch per kind over all members, then the batched reverse
     sweep); the tapes keep the stage inputs; the cotangents must agree bit
-    for bit. Each member's gradient is
+    for bit. The batch autotuner picks one tile height, used by both. Each member's gradient is
     checked in one smooth direction near its gauges against fourth-order
     differences of its plain solver's replay."""
     import numpy as np
@@ -466,6 +466,10 @@ def batched_adjoint_leg(args):
         members.append(s)
         data.append((c, theta, nodes, dt0))
     opts = sgnx.RkOpts(atol=1e-10, rtol=1e-8, max_accepted=steps)
+    # one tile height for every member, chosen by the batch autotuner and used
+    # by both the one-by-one and the batched runs (as in the batched sweep)
+    b = sgnx.Batch(members)
+    rows, waves = b.autotune()
 
     def start(s, c, nodes, taped):
         s.upload(c.U0)
@@ -491,7 +495,6 @@ def batched_adjoint_leg(args):
     solo = [s.adjoint(None, G) for s, G in zip(members, Gs)]
     seq["reverse"] = time.perf_counter() - t0
     # one batch: taped run and reverse sweep
-    b = sgnx.Batch(members)
     bat = {}
     for s, (c, theta, nodes, dt0) in zip(members, data):
         start(s, c, nodes, True)
@@ -535,7 +538,7 @@ def batched_adjoint_leg(args):
     grad_seq = seq["taped"] + seq["reverse"]
     grad_bat = bat["taped"] + bat["reverse"]
     out = {"members": len(members), "nx_min": min(sizes), "nx_max": max(sizes),
-           "nodes": nodes_total, "steps_per_member": steps,
+           "nodes": nodes_total, "steps_per_member": steps, "rows": rows, "waves3_mask": waves,
            "sequential_s": seq, "batched_s": bat,
            "gradient_sequential_s": grad_seq, "gradient_batched_s": grad_bat,
            "gradient_speedup": grad_seq / grad_bat,

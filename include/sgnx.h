@@ -143,7 +143,8 @@ int sgnx_bathymetry(const sgnx_solver* s, double* Hx, double* Hy);
 
 /* RHS kernel choice (tuning, no counterpart in the reference): variant 0 =
  * one workgroup per 10-element row tile, 1 = row sweep over a strip of 10
- * element columns and `rows` element rows per workgroup (default 1, rows 16;
+ * element columns and `rows` element rows per workgroup (default 1, rows 16,
+ * or fewer on grids too small to give every compute unit three workgroups;
  * the environment variables SGNX_RHS_VARIANT / SGNX_SWEEP_ROWS set the
  * defaults at creation). Both are bitwise equal to the reference rhs(). */
 int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows);

@@ -176,6 +176,19 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   } while (0)
 
   CK(hipGetDevice(&s->device));
+  if (!std::getenv("SGNX_SWEEP_ROWS")) {
+    // default tile height: the tallest that still gives every compute unit
+    // about three workgroups (small grids would leave most of the chip idle
+    // at 16 rows). The RHS does not depend on it; the adaptive step's error
+    // norm is summed per workgroup, so its last bits can.
+    int cus = 0;
+    CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    const long long strips = (d->Nx + 9) / 10, want = 3LL * std::max(cus, 1);
+    for (int r : {16, 12, 8, 6, 4, 3, 2}) {
+      g.rows = r;
+      if (strips * ((d->Ny + r - 1) / r) >= want) break;
+    }
+  }
   const size_t Nd = (size_t)g.Nd, nst = (size_t)NV * Nd;
   const size_t nfx = (size_t)(d->Nx + 1) * d->Ny * NP, nfy = (size_t)d->Nx * (d->Ny + 1) * NP;
   OpTables T;
