@@ -7,6 +7,7 @@
 #include "sgnx/sgn2d.hpp"
 
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <vector>
 
@@ -57,6 +58,21 @@ int main() {
     sgnx::State U0b;
     std::vector<double> thb;
     dev.adjoint(&ub, &wg, U0b, &thb);
+    // the same run taped with its stage inputs kept: the same cotangents, bit for bit
+    dev.upload(U0);
+    dev.setGauges(gauges, steps);
+    dev.tapeKeepStages(true);
+    dev.tapeBegin(1 << 20);
+    dev.integrateAdaptive(0.0, 1e9, dt0, opt);
+    dev.tapeEnd();
+    dev.tapeKeepStages(false);
+    sgnx::State U0k;
+    std::vector<double> thk;
+    dev.adjoint(&ub, &wg, U0k, &thk);
+    const bool keptSame =
+        U0k.size() == U0b.size() && thk.size() == thb.size() &&
+        std::memcmp(U0k.data(), U0b.data(), U0b.size() * sizeof(double)) == 0 &&
+        std::memcmp(thk.data(), thb.data(), thb.size() * sizeof(double)) == 0;
     double pU = 0.0, pT = 0.0;
     for (int i = 0; i < Nd; ++i) {
       pU += U0b[i] * dh[i];
@@ -85,10 +101,10 @@ int main() {
                       (12.0 * eps);
     const double err = std::fabs(pU + pT - fd) / (std::fabs(pU) + std::fabs(pT));
     std::printf("steps %ld, gauge rows %zu; adjoint %.12e (state %.3e, depth %.3e), "
-                "differences %.12e, rel err %.2e\n",
-                st.accepted, G.size() / gauges.size(), pU + pT, pU, pT, fd, err);
+                "differences %.12e, rel err %.2e; kept stages same bits %d\n",
+                st.accepted, G.size() / gauges.size(), pU + pT, pU, pT, fd, err, (int)keptSame);
     const bool ok = st.accepted == steps && G.size() == (size_t)steps * gauges.size() &&
-                    err < 1e-6;
+                    err < 1e-6 && keptSame;
     std::printf(ok ? "ADJOINT_OK\n" : "ADJOINT_FAIL\n");
     return ok ? 0 : 1;
   } catch (const sgnx::Error& e) {
