@@ -64,6 +64,61 @@ __device__ __forceinline__ void acc_add(double& s, double& c, double x) {
   c = c + e;
 }
 
+// Fixed-order reductions of (s, c, m) triples: a compensated sum s + c and a
+// max m. Within a wavefront, five butterfly levels of swizzles (lane ^ 1, 2,
+// 4, 8, 16 inside each half) and lane 32's triple read into lane 0; then the
+// waves' triples in wave order. Each level merges the partner's triple into
+// the lane's own in the same way, so lane 0's result is one fixed tree and
+// the same bits on every run.
+template <int XOR>
+__device__ __forceinline__ double swz_xor(double x) {
+  constexpr int kPat = (XOR << 10) | 0x1F;   // bit mode: and 0x1f, or 0, xor XOR
+  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(x), kPat);
+  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(x), kPat);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ void merge3(double& s, double& c, double& m, double s2, double c2,
+                                       double m2) {
+  acc_add(s, c, s2);
+  c = c + c2;
+  m = fmax(m, m2);
+}
+
+template <int XOR>
+__device__ __forceinline__ void wave_level(double& s, double& c, double& m) {
+  const double s2 = swz_xor<XOR>(s), c2 = swz_xor<XOR>(c), m2 = swz_xor<XOR>(m);
+  merge3(s, c, m, s2, c2, m2);
+}
+
+__device__ __forceinline__ double lane32(double x) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), 32);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), 32);
+  return __hiloint2double(hi, lo);
+}
+
+// Every thread of the block calls it (no divergence); the result is in
+// thread 0's s, c, m. ws: 3 doubles of LDS per wave.
+__device__ __forceinline__ void block_reduce(double& s, double& c, double& m, double* ws) {
+  wave_level<1>(s, c, m);
+  wave_level<2>(s, c, m);
+  wave_level<4>(s, c, m);
+  wave_level<8>(s, c, m);
+  wave_level<16>(s, c, m);
+  const double s2 = lane32(s), c2 = lane32(c), m2 = lane32(m);
+  merge3(s, c, m, s2, c2, m2);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    ws[3 * w] = s;
+    ws[3 * w + 1] = c;
+    ws[3 * w + 2] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) merge3(s, c, m, ws[3 * k], ws[3 * k + 1], ws[3 * k + 2]);
+  }
+}
+
 // Ghost state across a non-periodic side.
 __device__ __forceinline__ void wall_ghost(const double* In, int normal, double* G) {
 #pragma unroll
@@ -754,28 +809,12 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
   }
 
   if (MODE == kErr) {
-    // fixed-order reduction over the workgroup (LDS reused)
-    double* ss = &sF[0][0];
-    double* sc = &sF[1][0];
-    double* sm = &sF[2][0];
-    ss[t] = eS;
-    sc[t] = eC;
-    sm[t] = eM;
-    __syncthreads();
-    for (int off = NT / 2; off > 0; off >>= 1) {
-      if (t < off) {
-        double s = ss[t], c = sc[t];
-        acc_add(s, c, ss[t + off]);
-        ss[t] = s;
-        sc[t] = c + sc[t + off];
-        sm[t] = fmax(sm[t], sm[t + off]);
-      }
-      __syncthreads();
-    }
+    // fixed-order reduction over the workgroup (LDS of the last row reused)
+    block_reduce(eS, eC, eM, &sF[0][0]);
     if (t == 0) {
-      sa.part[3 * tile] = ss[0];
-      sa.part[3 * tile + 1] = sc[0];
-      sa.part[3 * tile + 2] = sm[0];
+      sa.part[3 * tile] = eS;
+      sa.part[3 * tile + 1] = eC;
+      sa.part[3 * tile + 2] = eM;
     }
   }
 }
@@ -856,29 +895,13 @@ k_rk4_update(double* __restrict__ u, double dt, const double* __restrict__ k1,
     u[i] += d6 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
 }
 
-// Fixed-order block reduction of (s, c, m) triples held in LDS.
-__device__ __forceinline__ void block_reduce(double* ss, double* sc, double* sm) {
-  const int t = threadIdx.x;
-  __syncthreads();
-  for (int off = kThreads / 2; off > 0; off >>= 1) {
-    if (t < off) {
-      double s = ss[t], c = sc[t];
-      acc_add(s, c, ss[t + off]);
-      ss[t] = s;
-      sc[t] = c + sc[t + off];
-      sm[t] = fmax(sm[t], sm[t + off]);
-    }
-    __syncthreads();
-  }
-}
-
 // rk.cpp:118-131: r = (uNew - uEmb) / (atol + rtol*max(|u|, |uNew|)).
 __global__ void __launch_bounds__(kThreads)
 k_err_norm(const double* __restrict__ u, const double* __restrict__ uNew,
            const double* __restrict__ k1, const double* __restrict__ k2,
            const double* __restrict__ k3, const double* __restrict__ k4, double dt, double atol,
            double rtol, size_t n, double* __restrict__ part) {
-  __shared__ double ss[kThreads], sc[kThreads], sm[kThreads];
+  __shared__ double ws[3 * kThreads / 64];
   double s = 0.0, c = 0.0, m = 0.0;
   const size_t stride = (size_t)gridDim.x * kThreads;
   for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
@@ -892,14 +915,11 @@ k_err_norm(const double* __restrict__ u, const double* __restrict__ uNew,
     const double ar = fabs(r);
     m = (m < ar) ? ar : m;                                     // std::max(errMax, |r|)
   }
-  ss[threadIdx.x] = s;
-  sc[threadIdx.x] = c;
-  sm[threadIdx.x] = m;
-  block_reduce(ss, sc, sm);
+  block_reduce(s, c, m, ws);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = ss[0];
-    part[3 * blockIdx.x + 1] = sc[0];
-    part[3 * blockIdx.x + 2] = sm[0];
+    part[3 * blockIdx.x] = s;
+    part[3 * blockIdx.x + 1] = c;
+    part[3 * blockIdx.x + 2] = m;
   }
 }
 
@@ -907,20 +927,17 @@ k_err_norm(const double* __restrict__ u, const double* __restrict__ uNew,
 // out[0] = sum, out[1] = max.
 __device__ __forceinline__ void finish_block(const double* __restrict__ part, int nb,
                                              double* __restrict__ out) {
-  __shared__ double ss[kThreads], sc[kThreads], sm[kThreads];
+  __shared__ double ws[3 * kThreads / 64];
   double s = 0.0, c = 0.0, m = 0.0;
   for (int i = threadIdx.x; i < nb; i += kThreads) {
     acc_add(s, c, part[3 * i]);
     c = c + part[3 * i + 1];
     m = fmax(m, part[3 * i + 2]);
   }
-  ss[threadIdx.x] = s;
-  sc[threadIdx.x] = c;
-  sm[threadIdx.x] = m;
-  block_reduce(ss, sc, sm);
+  block_reduce(s, c, m, ws);
   if (threadIdx.x == 0) {
-    out[0] = ss[0] + sc[0];
-    out[1] = sm[0];
+    out[0] = s + c;
+    out[1] = m;
   }
 }
 
@@ -1026,7 +1043,7 @@ k_elem_means(const double* __restrict__ U, const double* __restrict__ Hn,
 // integrand qw_a qw_b h (:319-332), 2 = energy integrand (:334-350).
 __global__ void __launch_bounds__(kThreads)
 k_diag(const double* __restrict__ U, Geometry g, Phys ph, int which, double* __restrict__ part) {
-  __shared__ double ss[kThreads], sc[kThreads], sm[kThreads];
+  __shared__ double ws[3 * kThreads / 64];
   double s = 0.0, c = 0.0, m = 0.0;
   const long long Nd = g.Nd;
   const size_t stride = (size_t)gridDim.x * kThreads;
@@ -1056,14 +1073,11 @@ k_diag(const double* __restrict__ U, Geometry g, Phys ph, int which, double* __r
       acc_add(s, c, val);
     }
   }
-  ss[threadIdx.x] = s;
-  sc[threadIdx.x] = c;
-  sm[threadIdx.x] = m;
-  block_reduce(ss, sc, sm);
+  block_reduce(s, c, m, ws);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = ss[0];
-    part[3 * blockIdx.x + 1] = sc[0];
-    part[3 * blockIdx.x + 2] = sm[0];
+    part[3 * blockIdx.x] = s;
+    part[3 * blockIdx.x + 1] = c;
+    part[3 * blockIdx.x + 2] = m;
   }
 }
 
