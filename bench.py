@@ -717,8 +717,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {max(args.gpus, world)}); "
-              "the multi-GPU decomposition is a design in DESIGN.md, not implemented",
+        print(f"bench.py: sgnx runs on one GPU only (asked for {max(args.gpus, world)}); "
+              "a decomposition over several GPUs is designed but not implemented",
               file=sys.stderr)
         sys.exit(2)
     if args.profile_rhs:

@@ -97,7 +97,7 @@ def test_python_mirror_raises_on_bad_arguments(sgnx_mod):
 
 def test_no_device_is_an_error_not_an_abort():
     """Creating a solver where no GPU is visible returns SGNX_EDEVICE (run in a
-    child process with the GPUs hidden, so it holds on a GPU box too)."""
+    child process with the GPUs hidden, so it holds on a machine with a GPU too)."""
     code = (
         "import sys; sys.path.insert(0, %r)\n"
         "import sgnx\n"
