@@ -3,7 +3,7 @@
 //
 // Compiled together with the reference's own sources (src/core/fr_operators.cpp,
 // src/solver/rk.cpp, src/solver/sgn2d.cpp, read where they lie) into
-// oracle/_ref/libsgnref.so by oracle/Makefile. It is the checker that pins the
+// oracle/_ref/libsgnref.so by oracle/Makefile. It is the reference build that pins the
 // oracle restatement: golden fixtures are written from it and, where it is
 // present, tests compare the oracle against it directly. Nothing here is part
 // of the product, and no reference source is copied into this repository.

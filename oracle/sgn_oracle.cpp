@@ -2,7 +2,7 @@
 // sgn_oracle.cpp -- CPU oracle: restatement of the reference 2D FR/SGN path.
 //
 // TEST INFRASTRUCTURE ONLY. Loaded by tests/, __graft_entry__.smoke() and the
-// CPU-baseline leg of bench.py as the checker; never linked by the product.
+// CPU-baseline leg of bench.py, to check results against; never linked by the product.
 //
 // Every floating-point expression below keeps the evaluation order of the
 // reference line it cites: the oracle's bits are the contract the HIP kernels

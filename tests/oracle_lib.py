@@ -1,4 +1,4 @@
-"""ctypes access to the CPU checkers (test infrastructure).
+"""ctypes access to the CPU oracle libraries (test infrastructure).
 
   oracle   oracle/liboracle.so       the C++ restatement of the reference path
   ref      oracle/_ref/libsgnref.so  the reference CPU solver built from its own
