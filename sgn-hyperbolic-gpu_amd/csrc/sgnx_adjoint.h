@@ -29,6 +29,15 @@ struct VjpArgs {
   double* out;
   double* thetaBar;
   double* coef;
+  // Optional (coefNext not null): the launch also writes the face
+  // coefficients of the step's next product, whose cotangent is
+  // can*An + cbn*out (out: this product's result) at the state Un, into
+  // coefNext -- what the coefficient kernel would write for it.
+  const double* Un;
+  const double* An;
+  double can, cbn;
+  double* coefNext;
+  bool coefReady;   // coef already holds this product's coefficients
 };
 
 inline long long vjp_coef_doubles(const Geometry& g) { return 120LL * g.Nx * g.Ny; }
@@ -60,7 +69,7 @@ struct GaugeMember {
 long long vjp_tiles(const Geometry& g);       // workgroups of one product launch
 long long comb_blocks(long long n);           // workgroups of a combination of n doubles
 hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart, int nMem,
-                                long long totalTiles, hipStream_t st);
+                                long long totalTiles, bool coefReady, hipStream_t st);
 hipError_t launch_comb_batch(const CombMember* mem, const long long* blockStart, int nMem,
                              long long totalBlocks, hipStream_t st);
 hipError_t launch_gauge_bar_batch(const GaugeMember* gm, int nMem, hipStream_t st);

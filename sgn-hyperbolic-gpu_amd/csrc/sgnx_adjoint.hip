@@ -420,6 +420,7 @@ __device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry&
   // phase 3: node adjoint
   const int jb = NPP * j, fb = 20 * j;
   double Hxb = 0.0, Hyb = 0.0, th = 0.0;
+  double outv[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};   // the node's result, for the next product
   if (isNode) {
     double dhx = 0.0, dhy = 0.0;
 #pragma unroll
@@ -487,23 +488,52 @@ __device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry&
       if (va.acc0) base = va.acc0[o];
       if (va.acc1) base = base + va.acc1[o];
       if (va.acc2) base = base + va.acc2[o];
-      va.out[o] = base + Ub[c];
+      outv[c] = base + Ub[c];
+      va.out[o] = outv[c];
     }
     th = Hlb + tb;
   }
-  if (!va.thetaBar) return;   // uniform over the block
-  // still depth: Hnod = theta, Hx = (D theta)/Jx, Hy = (D theta)/Jy per line
-  __syncthreads();
-  if (isNode) {
-    L.DX[t] = Hxb * iJx;
-    L.DY[t] = Hyb * iJy;
-  }
-  __syncthreads();
-  if (isNode) {
+  if (va.thetaBar) {   // uniform over the block
+    // still depth: Hnod = theta, Hx = (D theta)/Jx, Hy = (D theta)/Jy per line
+    __syncthreads();
+    if (isNode) {
+      L.DX[t] = Hxb * iJx;
+      L.DY[t] = Hyb * iJy;
+    }
+    __syncthreads();
+    if (isNode) {
 #pragma unroll
-    for (int k = 0; k < NP; ++k)
-      th = th + (sD[k * NP + a] * L.DX[jb + k + NP * b] + sD[k * NP + b] * L.DY[jb + a + NP * k]);
-    va.thetaBar[i] += th;
+      for (int k = 0; k < NP; ++k)
+        th = th + (sD[k * NP + a] * L.DX[jb + k + NP * b] + sD[k * NP + b] * L.DY[jb + a + NP * k]);
+      va.thetaBar[i] += th;
+    }
+  }
+  if (va.coefNext) {   // uniform over the block
+    // the next product's cotangent can*An + cbn*out on this tile, staged as
+    // tile_stage stages a product's own, then its face coefficients as
+    // vjp_coef_tile forms them
+    __syncthreads();
+    if (isNode) {
+      double Wn[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        Wn[c] = va.can * va.An[(long long)c * Nd + i];
+        Wn[c] = Wn[c] + va.cbn * outv[c];
+        L.W[c][t] = Wn[c];
+      }
+      const double q = Wn[IHP] * ph.c2 / va.Un[i];
+      L.DX[t] = q * va.Un[(long long)IHU * Nd + i] * (1.0 / g.Jx);
+      L.DY[t] = q * va.Un[(long long)IHV * Nd + i] * (1.0 / g.Jy);
+    }
+    __syncthreads();
+    if (isFace) {
+      double Ln[NV], ln;
+      face_coef(L, NPP * jf, f, n, iJx, iJy, Ln, ln);
+      double* cp = va.coefNext + (((long long)ey * Nx + ex0) * 20 + t) * 6;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) cp[c] = Ln[c];
+      cp[NV] = ln;
+    }
   }
 }
 
@@ -622,9 +652,11 @@ __global__ void k_gauge_bar(const double* __restrict__ Gb, const long long* __re
 hipError_t launch_rhs_vjp(const Geometry& g, const Phys& ph, const VjpArgs& va, GhostTime gt,
                           hipStream_t st) {
   const long long nb = (long long)g.Ny * ((g.Nx + kEW - 1) / kEW);
-  hipLaunchKernelGGL(k_vjp_coef, dim3((unsigned)nb), dim3(kT), 0, st, va, g, ph);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (!va.coefReady) {
+    hipLaunchKernelGGL(k_vjp_coef, dim3((unsigned)nb), dim3(kT), 0, st, va, g, ph);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_vjp_elem, dim3((unsigned)nb), dim3(kT), 0, st, va, g, ph, gt);
   return hipGetLastError();
 }
@@ -634,12 +666,14 @@ long long vjp_tiles(const Geometry& g) { return (long long)g.Ny * ((g.Nx + kEW -
 long long comb_blocks(long long n) { return (n + kCombChunk - 1) / kCombChunk; }
 
 hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart, int nMem,
-                                long long totalTiles, hipStream_t st) {
+                                long long totalTiles, bool coefReady, hipStream_t st) {
   if (totalTiles <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_vjp_coef_batch, dim3((unsigned)totalTiles), dim3(kT), 0, st, mem,
-                     tileStart, nMem);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (!coefReady) {
+    hipLaunchKernelGGL(k_vjp_coef_batch, dim3((unsigned)totalTiles), dim3(kT), 0, st, mem,
+                       tileStart, nMem);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_vjp_elem_batch, dim3((unsigned)totalTiles), dim3(kT), 0, st, mem,
                      tileStart, nMem);
   return hipGetLastError();

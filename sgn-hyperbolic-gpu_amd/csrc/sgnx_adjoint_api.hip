@@ -32,6 +32,16 @@
 
 #include "sgnx_tape.h"
 
+void chain_coef(VjpArgs& a, VjpArgs& b, double* buf) {
+  a.coefNext = buf;
+  a.Un = b.U;
+  a.An = b.A;
+  a.can = b.ca;
+  a.cbn = b.cb;
+  b.coef = buf;
+  b.coefReady = true;
+}
+
 namespace {
 
 int fail(sgnx_solver* s, int code, const std::string& msg) {
@@ -112,8 +122,29 @@ int bs3_fixed(sgnx_solver* s, const double* u, double* uNew, double t, double dt
   return SGNX_OK;
 }
 
-// lam <- (d u_{n+1} / d u_n)^T lam for the step u -> u_{n+1} of size dt at t;
-// thetaBar += the still-depth part. Uses d_k1, d_k2, d_k3, d_tmp.
+// lam <- (d u_{n+1} / d u_n)^T lam for the step u -> u_{n+1} of size dt at t,
+// whose stage inputs are y2, y3; thetaBar += the still-depth part. The three
+// products are chained: each of the first two also writes the next one's face
+// coefficients (tp->coef2, then tp->coef).
+//   uNew = u + dt ((2/9 k1 + 1/3 k2) + 4/9 k3)   (rk.cpp:106-108)
+int step_adjoint_stored(sgnx_solver* s, AdjTape* tp, const double* u, const double* y2,
+                        const double* y3, double t, double dt, double* thetaBar) {
+  int rc;
+  VjpArgs a3 = vjp_args(tp, y3, tp->lam, dt * (4.0 / 9.0), nullptr, 0.0, tp->y3b, thetaBar);
+  VjpArgs a2 = vjp_args(tp, y2, tp->lam, dt * (1.0 / 3.0), tp->y3b, 0.75 * dt, tp->y2b, thetaBar);
+  VjpArgs a1 = vjp_args(tp, u, tp->lam, dt * (2.0 / 9.0), tp->y2b, 0.5 * dt, tp->lam, thetaBar);
+  a1.acc0 = tp->lam;
+  a1.acc1 = tp->y3b;
+  a1.acc2 = tp->y2b;
+  chain_coef(a3, a2, tp->coef2);
+  chain_coef(a2, a1, tp->coef);
+  if ((rc = vjp(s, a3, t + 0.75 * dt))) return rc;
+  if ((rc = vjp(s, a2, t + 0.5 * dt))) return rc;
+  return vjp(s, a1, t);
+}
+
+// step_adjoint_stored with the stage inputs recomputed from u.
+// Uses d_k1, d_k2, d_k3, d_tmp.
 int step_adjoint(sgnx_solver* s, AdjTape* tp, const double* u, double t, double dt,
                  double* thetaBar) {
   const size_t n = s->n();
@@ -123,37 +154,13 @@ int step_adjoint(sgnx_solver* s, AdjTape* tp, const double* u, double t, double 
   HIPCHK(s, launch_comb1(y2, u, 0.5 * dt, s->d_k1, n, s->stream));
   HIPCHK(s, launch_rhs(s->g, s->ph, y2, s->d_k2, ghost_time(s, t + 0.5 * dt), s->stream));
   HIPCHK(s, launch_comb1(y3, u, 0.75 * dt, s->d_k2, n, s->stream));
-  int rc;
-  // uNew = u + dt ((2/9 k1 + 1/3 k2) + 4/9 k3)   (rk.cpp:106-108)
-  VjpArgs a3 = vjp_args(tp, y3, tp->lam, dt * (4.0 / 9.0), nullptr, 0.0, tp->y3b, thetaBar);
-  if ((rc = vjp(s, a3, t + 0.75 * dt))) return rc;
-  VjpArgs a2 = vjp_args(tp, y2, tp->lam, dt * (1.0 / 3.0), tp->y3b, 0.75 * dt, tp->y2b, thetaBar);
-  if ((rc = vjp(s, a2, t + 0.5 * dt))) return rc;
-  VjpArgs a1 = vjp_args(tp, u, tp->lam, dt * (2.0 / 9.0), tp->y2b, 0.5 * dt, tp->lam, thetaBar);
-  a1.acc0 = tp->lam;
-  a1.acc1 = tp->y3b;
-  a1.acc2 = tp->y2b;
-  return vjp(s, a1, t);
-}
-
-// step_adjoint with the stage inputs y2, y3 the taped run kept
-int step_adjoint_stored(sgnx_solver* s, AdjTape* tp, const double* u, const double* y2,
-                        const double* y3, double t, double dt, double* thetaBar) {
-  int rc;
-  VjpArgs a3 = vjp_args(tp, y3, tp->lam, dt * (4.0 / 9.0), nullptr, 0.0, tp->y3b, thetaBar);
-  if ((rc = vjp(s, a3, t + 0.75 * dt))) return rc;
-  VjpArgs a2 = vjp_args(tp, y2, tp->lam, dt * (1.0 / 3.0), tp->y3b, 0.75 * dt, tp->y2b, thetaBar);
-  if ((rc = vjp(s, a2, t + 0.5 * dt))) return rc;
-  VjpArgs a1 = vjp_args(tp, u, tp->lam, dt * (2.0 / 9.0), tp->y2b, 0.5 * dt, tp->lam, thetaBar);
-  a1.acc0 = tp->lam;
-  a1.acc1 = tp->y3b;
-  a1.acc2 = tp->y2b;
-  return vjp(s, a1, t);
+  return step_adjoint_stored(s, tp, u, y2, y3, t, dt, thetaBar);
 }
 
 int ensure_vjp_buffers(sgnx_solver* s, AdjTape* tp) {
   int rc;
   if ((rc = alloc_state(s, &tp->coef, (size_t)vjp_coef_doubles(s->g)))) return rc;
+  if ((rc = alloc_state(s, &tp->coef2, (size_t)vjp_coef_doubles(s->g)))) return rc;
   if ((rc = alloc_state(s, &tp->thb, (size_t)s->g.Nd))) return rc;
   return SGNX_OK;
 }
@@ -430,7 +437,8 @@ void tape_free(sgnx_solver* s) {
   release_stages(tp);
   for (double* p : tp->spare) (void)hipFree(p);
   for (double* p : tp->seg) (void)hipFree(p);
-  for (double* p : {tp->lam, tp->y3b, tp->y2b, tp->coef, tp->thb, tp->gb, tp->pp[0], tp->pp[1]})
+  for (double* p : {tp->lam, tp->y3b, tp->y2b, tp->coef, tp->coef2, tp->thb, tp->gb, tp->pp[0],
+                    tp->pp[1]})
     if (p) (void)hipFree(p);
   delete tp;
   s->tape = nullptr;

@@ -619,6 +619,8 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
       a1.acc0 = T->lam;
       a1.acc1 = T->y3b;
       a1.acc2 = T->y2b;
+      chain_coef(a3, a2, T->coef2);
+      chain_coef(a2, a1, T->coef);
       hV3[A] = VjpMember{a3, s->g, s->ph, ghost_time(s, t + 0.75 * dt)};
       hV2[A] = VjpMember{a2, s->g, s->ph, ghost_time(s, t + 0.5 * dt)};
       hV1[A] = VjpMember{a1, s->g, s->ph, ghost_time(s, t)};
@@ -644,9 +646,11 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
       BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC3)), dCT, Rn, cAcc,
                                 b->stream));
     }
+    // the products chained: the first launches the coefficient kernel, each
+    // later one finds its coefficients written by the one before
     for (size_t o : {oV3, oV2, oV1})
       BCHK(b, launch_rhs_vjp_batch(reinterpret_cast<const VjpMember*>(dptr(o)), dVT, A, vAcc,
-                                   b->stream));
+                                   o != oV3, b->stream));
     // the arena is refilled for the next iteration
     BCHK(b, hipStreamSynchronize(b->stream));
   }

@@ -24,6 +24,7 @@ struct AdjTape {
   std::vector<long long> gnodes;   // gauge nodes while recording
   // reverse-sweep buffers
   double *lam = nullptr, *y3b = nullptr, *y2b = nullptr, *coef = nullptr, *thb = nullptr;
+  double* coef2 = nullptr;         // second face-coefficient buffer (chained products)
   double* gb = nullptr;            // gauge cotangents staged from the host
   size_t gbCap = 0;
   long slots = -1;                 // states the sweep may hold besides the snapshots (-1: any)
@@ -49,5 +50,9 @@ int adjoint_inputs(sgnx_solver* s, AdjTape* tp, const double* Ubar_final, const 
 // On stream st: U0bar = lam, theta_bar = thetaBar (when given).
 int adjoint_outputs(sgnx_solver* s, AdjTape* tp, double* U0bar, double* theta_bar, int mem,
                     hipStream_t st);
+// Chains product a to the step's next product b (b's cotangent is
+// b.ca*b.A + b.cb*a.out): a's launch also writes b's face coefficients into
+// buf, and b's launch skips the coefficient kernel.
+void chain_coef(VjpArgs& a, VjpArgs& b, double* buf);
 
 #endif  // SGNX_TAPE_H

@@ -283,6 +283,36 @@ def test_kept_stages_give_the_same_adjoint(ci):
         assert_bitwise(tb, tb0, f"{tag}: thetaBar")
 
 
+@pytest.mark.parametrize("ci,keep", [(0, True), (3, True), (3, False)])
+def test_chained_step_equals_single_products_bitwise(ci, keep):
+    """The reverse step chains its three products: the first two launches also
+    write the face coefficients of the next product's cotangent, and the next
+    launch skips the coefficient kernel. One reversed step equals the same
+    three products evaluated one at a time through sgnx_rhs_vjp, with the
+    cotangents and sums formed on the host in the kernels' order, bit for bit
+    (stage inputs kept by the taped run, or recomputed by the sweep)."""
+    c = _cases()[ci]
+    rng = np.random.default_rng(900 + ci)
+    theta = c.depth["Hnod"] + 0.002 * rng.uniform(-1, 1, c.nd)
+    U0 = c.U0 + 1e-3 * rng.uniform(-1, 1, c.U0.size) * np.repeat([1, 0.5, 0.5, 0.2, 0.2], c.nd)
+    nodes = rng.choice(c.nd, size=3, replace=False).astype(np.int64)
+    s, UN, _ = _taped_run(c, theta, U0, nodes, 1, 1 << 20, keep)
+    lam = rng.normal(size=UN.size)
+    U0b, thb = s.adjoint(lam, None)
+    (t,), (dt,) = s.tapeSteps()
+    # the step's stage inputs (rk.cpp:99-104), formed as the fused launches form them
+    k1 = s.rhs(U0, t)
+    y2 = U0 + (0.5 * dt) * k1
+    k2 = s.rhs(y2, t + 0.5 * dt)
+    y3 = U0 + (0.75 * dt) * k2
+    y3b, th3 = s.rhs_vjp(y3, (dt * (4.0 / 9.0)) * lam, t + 0.75 * dt)
+    y2b, th2 = s.rhs_vjp(y2, (dt * (1.0 / 3.0)) * lam + (0.75 * dt) * y3b, t + 0.5 * dt)
+    p1, th1 = s.rhs_vjp(U0, (dt * (2.0 / 9.0)) * lam + (0.5 * dt) * y2b, t)
+    s.close()
+    assert_bitwise(U0b, ((lam + y3b) + y2b) + p1, "U0bar of one chained step")
+    assert_bitwise(thb, ((0.0 + th3) + th2) + th1, "thetaBar of one chained step")
+
+
 @pytest.mark.parametrize("budget,keep", [(1 << 20, False), (3, False), (1 << 20, True)])
 def test_taped_run_leaves_the_solver_intact(budget, keep):
     """Snapshots take over the buffer of each accepted step's input instead of
