@@ -389,7 +389,10 @@ int tape_step_begin(sgnx_solver* s) {
   if (tp->keepStages && !tp->stagesOff && s->g.variant == 1 && (long)tp->y2s.size() == n) {
     double *a = nullptr, *b = nullptr;
     if ((rc = take_spare(s, tp, &a))) return rc;
-    if ((rc = take_spare(s, tp, &b))) return rc;
+    if ((rc = take_spare(s, tp, &b))) {
+      tp->spare.push_back(a);
+      return rc;
+    }
     tp->y2s.push_back(a);
     tp->y3s.push_back(b);
   }
