@@ -71,6 +71,43 @@ def test_rhs_random_bitwise(Nx, Ny, sides):
     assert_bitwise(got, want, f"random {Nx}x{Ny} {sides}")
 
 
+def test_rhs_manning_within_tolerance():
+    """Manning friction on (n = 0.03, sgn_flux2d.h:135-141): the friction
+    coefficient divides by cbrt(h), and the device's cbrt is the device
+    library's, not the C library's, so a friction-on RHS is held to a
+    tolerance: each entry within 1e-13 of the oracle's relative to its
+    variable's largest magnitude (a one-ulp cbrt difference moves the
+    friction term by about 1e-16 of itself). The trajectory bar holds too:
+    20 adaptive steps with the oracle's step counts and relative L2 <= 1e-12."""
+    import dataclasses
+    import sgnx
+    from sgnx import cases
+    c = cases.gate(Nx=11, Ny=7, bc=("periodic", "periodic", "wall", "wall"))
+    c = dataclasses.replace(c, par=cases.Params(c.par.g, c.par.c, c.par.gamma, 0.03))
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    o = OL.Oracle.from_case(c)
+    got, want = s.rhs(c.U0), o.rhs(c.U0)
+    nd = c.nd
+    for v in range(5):
+        g_, w_ = got[v * nd:(v + 1) * nd], want[v * nd:(v + 1) * nd]
+        scale = max(np.abs(w_).max(), 1e-300)
+        assert np.abs(g_ - w_).max() <= 1e-13 * scale, v
+    # the friction changes the result (the test is not vacuous)
+    o0 = OL.Oracle.from_case(dataclasses.replace(c, par=cases.Params(c.par.g, c.par.c, c.par.gamma)))
+    assert np.abs(o0.rhs(c.U0) - want).max() > 1e-6
+    o0.close()
+    dt0 = o.cfl_time_step(c.U0, 0.5)
+    s.upload(c.U0)
+    st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(atol=1e-10, rtol=1e-8, max_accepted=20))
+    Ug = s.download()
+    Uo, sto, _, _, _ = o.integrate(c.U0, 0.0, 1e9, dt0,
+                                   OL.OrcOpts(atol=1e-10, rtol=1e-8, max_accepted=20))
+    s.close()
+    o.close()
+    assert (st.accepted, st.rejected, st.rhsEvals) == (sto.accepted, sto.rejected, sto.rhsEvals)
+    assert np.linalg.norm(Ug - Uo) <= 1e-12 * np.linalg.norm(Uo)
+
+
 def test_rhs_flat_no_depth_bitwise():
     """Flat bottom given as no still depth at all (the reference's H == 0)."""
     from sgnx import cases
