@@ -523,14 +523,14 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
     if (rc) return bfail(b, rc, "batch adjoint, member " + std::to_string(i) + ": " + s->err);
     Nmax = std::max(Nmax, (long)tp[i]->t.size());
   }
-  // argument arena: gauges, two RHS tables + offsets, two combinations +
-  // offsets, three products + offsets
+  // argument arena: gauges, two RHS tables (R(u); the stage-2 launch keeping
+  // y2) + offsets, the y3 combination + offsets, three products + offsets
   auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t nn = (size_t)n;
   const size_t oG = 0, oR1 = up(oG + nn * sizeof(GaugeMember));
   const size_t oR2 = up(oR1 + nn * sizeof(BatchMember)), oRT = up(oR2 + nn * sizeof(BatchMember));
-  const size_t oC2 = up(oRT + (nn + 1) * sizeof(long long));
-  const size_t oC3 = up(oC2 + nn * sizeof(CombMember)), oCT = up(oC3 + nn * sizeof(CombMember));
+  const size_t oC3 = up(oRT + (nn + 1) * sizeof(long long));
+  const size_t oCT = up(oC3 + nn * sizeof(CombMember));
   const size_t oV3 = up(oCT + (nn + 1) * sizeof(long long));
   const size_t oV2 = up(oV3 + nn * sizeof(VjpMember)), oV1 = up(oV2 + nn * sizeof(VjpMember));
   const size_t oVT = up(oV1 + nn * sizeof(VjpMember));
@@ -549,7 +549,6 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
   auto* hR1 = reinterpret_cast<BatchMember*>(b->h_adj + oR1);
   auto* hR2 = reinterpret_cast<BatchMember*>(b->h_adj + oR2);
   auto* hRT = reinterpret_cast<long long*>(b->h_adj + oRT);
-  auto* hC2 = reinterpret_cast<CombMember*>(b->h_adj + oC2);
   auto* hC3 = reinterpret_cast<CombMember*>(b->h_adj + oC3);
   auto* hCT = reinterpret_cast<long long*>(b->h_adj + oCT);
   auto* hV3 = reinterpret_cast<VjpMember*>(b->h_adj + oV3);
@@ -585,12 +584,15 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
         Stage s1{};
         s1.u = u;
         hR1[Rn] = member_args(s, u, s->d_k1, s1, t);
+        // the forward's stage-2 launch, keeping its input: y2 and k2 = R(y2)
         Stage s2{};
-        s2.u = s->d_tmp;
-        hR2[Rn] = member_args(s, s->d_tmp, s->d_k2, s2, t + 0.5 * dt);
+        s2.u = u;
+        s2.k1 = s->d_k1;
+        s2.a = 0.5 * dt;
+        s2.yOut = s->d_tmp;
+        hR2[Rn] = member_args(s, u, s->d_k2, s2, t + 0.5 * dt);
         hRT[Rn] = rAcc;
         rAcc += sweep_tiles(s->g);
-        hC2[Rn] = CombMember{s->d_tmp, u, 0.5 * dt, s->d_k1, nst};
         hC3[Rn] = CombMember{s->d_k3, u, 0.75 * dt, s->d_k2, nst};
         hCT[Rn] = cAcc;
         cAcc += comb_blocks(nst);
@@ -640,9 +642,8 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
     BCHK(b, launch_gauge_bar_batch(reinterpret_cast<const GaugeMember*>(dptr(oG)), A, b->stream));
     if (Rn > 0) {
       BCHK(b, launch_rhs_stage_batch(dR1, dRT, Rn, rAcc, kPlain, b->waves3 & 1, b->stream));
-      BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC2)), dCT, Rn, cAcc,
-                                b->stream));
-      BCHK(b, launch_rhs_stage_batch(dR2, dRT, Rn, rAcc, kPlain, b->waves3 & 1, b->stream));
+      BCHK(b, launch_rhs_stage_batch(dR2, dRT, Rn, rAcc, kAxpy, (b->waves3 >> 1) & 1, b->stream,
+                                     true));
       BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC3)), dCT, Rn, cAcc,
                                 b->stream));
     }
