@@ -36,6 +36,8 @@ Also reported:
   batched       the 64-member uneven sweep (Nx 128..768): one at a time against
                 one batch (one launch per stage kind, captured step graph),
                 members bit-identical
+  batched_small the same with 64 small members (Nx 32..128, 50 steps each), where
+                per-step launch and synchronisation costs dominate a solo run
   adjoint       a gradient at 1024x512 (taped run + reverse sweep) against the
                 plain run, with and without kept stage inputs
   inversion     the gauge-inversion gradient (256x128, 2000 steps, 16 snapshots +
@@ -167,7 +169,7 @@ def plain_rhs_rate(s, nd, n=20):
             "frac": gbs / HBM_PEAK_GBS}
 
 
-def batched_sweep(args):
+def batched_sweep(args, nx_min=128, nx_max=768, n_members=None, steps=None):
     """The batched shoal sweep: --batch-members grids of seeded uneven sizes
     (Nx in [128, 768], Ny = Nx/2; cases.sweep_member), each advanced
     --batch-steps accepted adaptive steps, first one at a time with the plain
@@ -179,8 +181,10 @@ def batched_sweep(args):
     import hashlib
     import sgnx
     from sgnx import cases
+    n_members = args.batch_members if n_members is None else n_members
+    steps = args.batch_steps if steps is None else steps
     t = time.perf_counter()
-    sizes = cases.sweep_sizes(args.batch_members, seed=args.seed)
+    sizes = cases.sweep_sizes(n_members, seed=args.seed, nx_min=nx_min, nx_max=nx_max)
     members, U0s, dt0s = [], [], []
     for i, nx in enumerate(sizes):
         c = cases.sweep_member(args.seed * 1000 + i + 1, nx)
@@ -192,7 +196,7 @@ def batched_sweep(args):
     nodes = sum(s.nDof() for s in members)
     log(f"[bench] batch: {len(members)} members, Nx {min(sizes)}..{max(sizes)}, "
         f"{nodes / 1e6:.1f} M nodes, built in {time.perf_counter() - t:.1f}s")
-    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=args.batch_steps)
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=steps)
     one = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=1)
     # the members one at a time with their own default tile heights (the
     # tallest that fills the chip, per grid): the sequential rate a user of
@@ -233,7 +237,7 @@ def batched_sweep(args):
     for s in members:
         s.close()
     out = {"members": len(members), "nx_min": min(sizes), "nx_max": max(sizes),
-           "nodes": nodes, "steps_per_member": args.batch_steps,
+           "nodes": nodes, "steps_per_member": steps,
            "autotune": {"rows": rows, "waves3": waves},
            "sequential": {"value": seq_dof / seq_wall, "wall_s": seq_wall},
            "sequential_own_tiles": {"value": own_dof / own_wall, "wall_s": own_wall},
@@ -696,6 +700,9 @@ def main():
                     help="members of the batched sweep leg (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=10,
                     help="accepted steps per member in the batched sweep leg")
+    ap.add_argument("--batch-small-members", type=int, default=64,
+                    help="members of the small-grid sweep leg, Nx in [32, 128] (0 = skip)")
+    ap.add_argument("--batch-small-steps", type=int, default=50)
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="accepted steps of the gradient leg (0 = skip)")
     ap.add_argument("--inversion-steps", type=int, default=2000,
@@ -795,6 +802,16 @@ def main():
             batched = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] batch leg failed: {batched['error']}")
 
+    batched_small = None
+    if args.batch_small_members > 0:
+        try:
+            batched_small = batched_sweep(args, nx_min=32, nx_max=128,
+                                          n_members=args.batch_small_members,
+                                          steps=args.batch_small_steps)
+        except Exception as e:                    # reported, not fatal to the main line
+            batched_small = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] small-grid batch leg failed: {batched_small['error']}")
+
     adjoint = None
     if args.adjoint_steps > 0:
         try:
@@ -871,6 +888,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_small": batched_small,
         "adjoint": adjoint,
         "inversion": inversion,
         "batched_adjoint": batched_adjoint,
