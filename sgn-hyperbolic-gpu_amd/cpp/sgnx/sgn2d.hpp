@@ -215,6 +215,12 @@ class SGN2DHip {
   // keep each taped step's stage inputs (two more states per step) so the
   // reverse sweep recomputes no stage; set before the run
   void tapeKeepStages(bool on) { detail::check(sgnx_tape_keep_stages(s_, on ? 1 : 0), s_); }
+  // steps whose stage inputs the tape keeps
+  long tapeStages() const {
+    long n = 0;
+    detail::check(sgnx_tape_stages(s_, &n), s_);
+    return n;
+  }
   // the recorded steps again (same t, dt) from the resident state
   void tapeReplay() { detail::check(sgnx_tape_replay(s_), s_); }
   // cotangents of the recorded run's initial state and still depth

@@ -66,11 +66,12 @@ int main() {
     dev.integrateAdaptive(0.0, 1e9, dt0, opt);
     dev.tapeEnd();
     dev.tapeKeepStages(false);
+    const long kept = dev.tapeStages();
     sgnx::State U0k;
     std::vector<double> thk;
     dev.adjoint(&ub, &wg, U0k, &thk);
     const bool keptSame =
-        U0k.size() == U0b.size() && thk.size() == thb.size() &&
+        kept == steps && U0k.size() == U0b.size() && thk.size() == thb.size() &&
         std::memcmp(U0k.data(), U0b.data(), U0b.size() * sizeof(double)) == 0 &&
         std::memcmp(thk.data(), thb.data(), thb.size() * sizeof(double)) == 0;
     double pU = 0.0, pT = 0.0;
