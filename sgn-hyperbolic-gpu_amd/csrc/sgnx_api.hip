@@ -183,7 +183,7 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
     // norm is summed per workgroup, so its last bits can.
     int cus = 0;
     CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
-    const long long strips = (d->Nx + 9) / 10, want = 3LL * std::max(cus, 1);
+    const long long strips = (d->Nx + kStripW - 1) / kStripW, want = 3LL * std::max(cus, 1);
     for (int r : {16, 12, 8, 6, 4, 3, 2}) {
       g.rows = r;
       if (strips * ((d->Ny + r - 1) / r) >= want) break;
@@ -235,7 +235,7 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   {
     // partial triples: kReduceBlocks for the elementwise reductions, one per
     // workgroup of the fused candidate stage (at most strips * Ny)
-    const long long strips = (d->Nx + 9) / 10;
+    const long long strips = (d->Nx + kStripW - 1) / kStripW;
     s->partCap = 3 * std::max<long long>(kReduceBlocks, strips * d->Ny);
     CK(hipMalloc((void**)&s->d_part, (size_t)s->partCap * sizeof(double)));
   }

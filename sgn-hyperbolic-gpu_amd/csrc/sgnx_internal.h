@@ -25,6 +25,11 @@ struct OpTables {
   double qw[NP];
 };
 
+// Element columns per RHS tile / row-sweep strip. Five columns make a
+// 128-work-item workgroup (125 node items, two waves), so a compute unit holds
+// four workgroups of the two-wave builds and six of the three-wave one.
+constexpr int kStripW = 5;
+
 // Bijective remap of workgroup ids so that XCD x (ids x, x+8, x+16, ...)
 // receives the contiguous tile range [start_x, start_x + count_x).
 __device__ __forceinline__ int xcd_tile(int bid, int nb) {

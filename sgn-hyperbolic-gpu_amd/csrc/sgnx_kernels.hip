@@ -1086,7 +1086,7 @@ inline unsigned elementwise_blocks(size_t n) {
   return (unsigned)(want < 16384 ? (want ? want : 1) : 16384);
 }
 
-constexpr int kTileW = 10;   // elements per RHS tile / sweep strip
+constexpr int kTileW = kStripW;   // elements per RHS tile / sweep strip
 
 }  // namespace
 
