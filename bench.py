@@ -45,6 +45,9 @@ Also reported:
   batched_adjoint  16 uneven members' gradients, one by one against one batch,
                 each checked by differences
   margin        the 2000x1000 margin workload (50 M nodes), 1000 adaptive steps
+  margin_adjoint  a source-inversion gradient on the margin workload (100 steps,
+                10 snapshots + 10 recomputation slots of 2 GB each) checked by
+                differences in 3 directions
 
 usage: python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--seed 0]
        python bench.py --profile-rhs N    (N RHS launches + a copy of known size,
@@ -577,7 +580,15 @@ def batched_adjoint_leg(args):
     return out
 
 
-def margin_leg(args):
+def margin_case(args):
+    from sgnx import cases
+    t = time.perf_counter()
+    case = cases.margin(seed=args.seed)
+    log(f"[bench] margin case {case.Nx}x{case.Ny} built in {time.perf_counter() - t:.1f}s")
+    return case
+
+
+def margin_leg(args, case):
     """The large single-grid margin workload (cases.margin: 2000x1000
     elements, 50 M nodes, 2 GB per state, on 200x100 km; a tanh margin
     4000 -> 50 m with 2 % roughness, walls in x, periodic in y, an elliptical
@@ -585,10 +596,6 @@ def margin_leg(args):
     device-resident call after a short warm-up, the same rate as the main
     line, plus the plain RHS's rate on the final state."""
     import sgnx
-    from sgnx import cases
-    t = time.perf_counter()
-    case = cases.margin(seed=args.seed)
-    log(f"[bench] margin case {case.Nx}x{case.Ny} built in {time.perf_counter() - t:.1f}s")
     s = sgnx.SGN2D(**case.solver_kwargs())
     s.upload(case.U0)
     dt0 = s.cflTimeStep(None, 0.5)
@@ -615,6 +622,91 @@ def margin_leg(args):
             "dof_rhs_per_s": rate, "ms_per_step": 1e3 * wall / max(st.accepted, 1),
             "t_reached_s": st.tReached, "mass_rel_change": abs(m1 - m0) / abs(m0),
             "plain_rhs": plain}
+
+
+def margin_adjoint_leg(args, case):
+    """A source-inversion gradient on the margin workload: the gradient of
+    the gauge misfit 0.5 |G(eta0) - G_obs|^2 with respect to the initial
+    surface over --margin-adjoint-steps adaptive steps, 16 gauges inside the
+    uplift, G_obs the same run from 1.1 times the uplift. At 2 GB per state
+    the tape holds 10 snapshots and the sweep 10 more states (binomial
+    recomputation). Reports its cost against the plain run and the largest
+    relative difference to fourth-order central differences of the plain
+    solver's replay of the taped steps along --margin-adjoint-dirs seeded
+    smooth directions (Gaussians 2-6 km wide near the gauges)."""
+    import numpy as np
+    import sgnx
+    from sgnx import cases
+    nd, steps = case.nd, args.margin_adjoint_steps
+    Hn = case.depth["Hnod"]
+    eta0 = case.U0[:nd] - Hn
+    rest = case.U0[nd:]
+    nodes = cases.margin_gauges(case, 16, seed=args.seed)
+    s = sgnx.SGN2D(**case.solver_kwargs())
+    s.upload(case.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = sgnx.RkOpts(atol=case.meta["atol"], rtol=case.meta["rtol"], max_accepted=steps)
+    s.upload(np.concatenate([Hn + 1.1 * eta0, rest]))
+    s.setGauges(nodes, steps)
+    s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    G_obs = s.gaugeLog()[1]
+    s.upload(case.U0)
+    s.setGauges(nodes, steps)
+    t0 = time.perf_counter()
+    st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    t_fwd = time.perf_counter() - t0
+    s.upload(case.U0)
+    s.setGauges(nodes, steps)
+    s.tapeBegin(10)
+    t0 = time.perf_counter()
+    st1 = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    t_tape = time.perf_counter() - t0
+    s.tapeEnd()
+    s.tapeSlots(10)
+    G = s.gaugeLog()[1]
+    t0 = time.perf_counter()
+    g_eta = s.adjoint(None, G - G_obs, theta=False)[:nd]
+    t_adj = time.perf_counter() - t0
+    recomputed, peak = s.adjointStats()
+    _, stride, held = s.tapeInfo()
+
+    def misfit(e):
+        s.upload(np.concatenate([Hn + e, rest]))
+        s.setGauges(nodes, steps)
+        s.tapeReplay()
+        return 0.5 * np.sum((s.gaugeLog()[1] - G_obs) ** 2)
+
+    rng = np.random.default_rng(args.seed + 5)
+    eps, errs = 3e-3, []
+    t0 = time.perf_counter()
+    for _ in range(args.margin_adjoint_dirs):
+        de = cases.smooth_direction(case, rng, n=2, near=nodes, scale=1000.0)
+        f = {k: misfit(eta0 + k * eps * de) for k in (-2, -1, 1, 2)}
+        fd = (8.0 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12.0 * eps)
+        pe = g_eta @ de
+        errs.append(abs(pe - fd) / max(abs(pe), abs(fd), 1e-300))
+    t_fd = time.perf_counter() - t0
+    s.close()
+    log(f"[bench] margin adjoint: {st.accepted} steps, forward {1e3 * t_fwd:.0f} ms, taped "
+        f"{1e3 * t_tape:.0f} ms, reverse {1e3 * t_adj:.0f} ms ({held} snapshots every {stride}, "
+        f"{peak} slots, {recomputed} steps recomputed) -> gradient/forward "
+        f"{(t_tape + t_adj) / t_fwd:.2f}; FD over {len(errs)} directions: max rel err "
+        f"{max(errs) if errs else float('nan'):.2e} ({t_fd:.1f}s)")
+    return {"workload": (f"margin {case.Nx}x{case.Ny} (seed {args.seed}): gradient of the gauge "
+                         f"misfit w.r.t. the initial surface, {nodes.size} gauges in the uplift, "
+                         f"{steps} accepted steps, atol 1e-8 rtol 1e-6"),
+            "accepted": st.accepted, "rejected": st.rejected,
+            "same_step_counts": st.as_dict() == st1.as_dict(),
+            "stored_states": {"snapshots": held, "stride": stride, "slots_peak": peak,
+                              "budget": 20, "bytes_per_state": 5 * 8 * nd},
+            "recomputed_steps": recomputed,
+            "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "adjoint_ms": 1e3 * t_adj,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "gradient_over_forward": (t_tape + t_adj) / t_fwd,
+            "fd_directions": len(errs), "fd_max_rel_err": max(errs) if errs else None,
+            "fd_reference": ("fourth-order central differences (eps 3e-3) of the plain solver's "
+                             "replay of the taped steps, along seeded smooth directions (2 "
+                             "Gaussians 2-6 km wide near the gauges)")}
 
 
 def cpu_baseline(case, gpu_solver, dt0, args):
@@ -714,6 +806,10 @@ def main():
     ap.add_argument("--adjoint-member-steps", type=int, default=60)
     ap.add_argument("--margin-steps", type=int, default=1000,
                     help="accepted steps of the 2000x1000 margin leg (0 = skip)")
+    ap.add_argument("--margin-adjoint-steps", type=int, default=100,
+                    help="accepted steps of the margin source-inversion gradient (0 = skip)")
+    ap.add_argument("--margin-adjoint-dirs", type=int, default=3,
+                    help="finite-difference directions checked on the margin gradient")
     ap.add_argument("--profile-rhs", type=int, default=0)
     ap.add_argument("--profile-vjp", type=int, default=0,
                     help="profile mode: this many RHS vector-Jacobian products")
@@ -836,13 +932,22 @@ def main():
             batched_adjoint = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] batched adjoint leg failed: {batched_adjoint['error']}")
 
-    margin = None
-    if args.margin_steps > 0:
-        try:
-            margin = margin_leg(args)
-        except Exception as e:                    # reported, not fatal to the main line
-            margin = {"error": f"{type(e).__name__}: {e}"}
-            log(f"[bench] margin leg failed: {margin['error']}")
+    margin = margin_adjoint = None
+    if args.margin_steps > 0 or args.margin_adjoint_steps > 0:
+        mcase = margin_case(args)
+        if args.margin_steps > 0:
+            try:
+                margin = margin_leg(args, mcase)
+            except Exception as e:                # reported, not fatal to the main line
+                margin = {"error": f"{type(e).__name__}: {e}"}
+                log(f"[bench] margin leg failed: {margin['error']}")
+        if args.margin_adjoint_steps > 0:
+            try:
+                margin_adjoint = margin_adjoint_leg(args, mcase)
+            except Exception as e:                # reported, not fatal to the main line
+                margin_adjoint = {"error": f"{type(e).__name__}: {e}"}
+                log(f"[bench] margin adjoint leg failed: {margin_adjoint['error']}")
+        del mcase
 
     out = {
         "metric": METRIC,
@@ -893,6 +998,7 @@ def main():
         "inversion": inversion,
         "batched_adjoint": batched_adjoint,
         "margin": margin,
+        "margin_adjoint": margin_adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -142,7 +142,7 @@ int sgnx_energy(sgnx_solver* s, double* out);
 int sgnx_bathymetry(const sgnx_solver* s, double* Hx, double* Hy);
 
 /* RHS kernel choice (tuning, no counterpart in the reference): variant 0 =
- * one workgroup per 10-element row tile, 1 = row sweep over a strip of 10
+ * one workgroup per 5-element row tile, 1 = row sweep over a strip of 5
  * element columns and `rows` element rows per workgroup (default 1, rows 16,
  * or fewer on grids too small to give every compute unit three workgroups;
  * the environment variables SGNX_RHS_VARIANT / SGNX_SWEEP_ROWS set the

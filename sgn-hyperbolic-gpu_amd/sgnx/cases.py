@@ -373,22 +373,38 @@ def margin(seed=0, Nx=2000, Ny=1000, P=4, rough=0.02):
                       "margin": mp, "rough": rough})
 
 
-def smooth_direction(case, rng, n=4, amp=0.01, near=None):
-    """A seeded smooth nodal field (n Gaussians of width 2-6 m and amplitude
-    up to amp, continuous across elements): a perturbation direction for
-    finite-difference checks of gradients of runs whose recorded step sizes
-    are held fixed, where only resolved (smooth) perturbations stay bounded.
-    near: node indices (e.g. gauges) to centre the Gaussians on, so that the
-    direction reaches them within a short run."""
+def margin_gauges(case, n=16, seed=0):
+    """n seeded gauge nodes (ascending) inside the margin case's uplift
+    ellipse, where a short run's surface moves: the stations of a source
+    inversion over the first steps."""
+    mp = case.meta["margin"]
+    x, y = node_coords(case.P, case.Nx, case.Ny, case.Lx, case.Ly)
+    dx = (x - mp["xs"] * case.Lx) / mp["sx"]
+    dy = (y - mp["ys"] * case.Ly) / mp["sy"]
+    inside = np.flatnonzero(dx * dx + dy * dy < 1.0)
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(inside, size=min(n, inside.size), replace=False)).astype(np.int64)
+
+
+def smooth_direction(case, rng, n=4, amp=0.01, near=None, scale=1.0):
+    """A seeded smooth nodal field (n Gaussians of width 2-6 m times scale and
+    amplitude up to amp, continuous across elements): a perturbation direction
+    for finite-difference checks of gradients of runs whose recorded step
+    sizes are held fixed, where only resolved (smooth) perturbations stay
+    bounded. near: node indices (e.g. gauges) to centre the Gaussians on
+    (within 1 m times scale), so that the direction reaches them within a
+    short run. scale = 1 suits grids of metres (the shoal, the inversion);
+    the margin's 100 m elements take kilometres (scale 1000)."""
     x, y = node_coords(case.P, case.Nx, case.Ny, case.Lx, case.Ly)
     out = np.zeros_like(x)
     for _ in range(n):
         if near is not None:
             k = int(near[rng.integers(len(near))])
-            xc, yc = x[k] + rng.uniform(-1.0, 1.0), y[k] + rng.uniform(-1.0, 1.0)
+            xc = x[k] + scale * rng.uniform(-1.0, 1.0)
+            yc = y[k] + scale * rng.uniform(-1.0, 1.0)
         else:
             xc, yc = rng.uniform(0.0, case.Lx), rng.uniform(0.0, case.Ly)
-        a, w = rng.uniform(-amp, amp), rng.uniform(2.0, 6.0)
+        a, w = rng.uniform(-amp, amp), scale * rng.uniform(2.0, 6.0)
         out = out + a * np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / (w * w))
     return out
 

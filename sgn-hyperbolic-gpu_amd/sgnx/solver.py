@@ -268,7 +268,7 @@ class SGN2D:
         return self._diag(self.lib.sgnx_energy, U)
 
     def set_rhs_kernel(self, variant, rows=16):
-        """0: one workgroup per 10-element row tile; 1: row sweep of `rows` rows."""
+        """0: one workgroup per 5-element row tile; 1: row sweep of `rows` rows."""
         L.check(self.lib.sgnx_set_rhs_kernel(self.h, int(variant), int(rows)), self.h)
 
     # ---- diagnostics (eta = h - H) ---------------------------------------------

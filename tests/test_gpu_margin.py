@@ -101,3 +101,59 @@ def test_margin_fullsize_mass_conserved(margin_full):
     assert np.all(np.isfinite(U1))
     assert abs(m1 - m0) <= 1e-12 * abs(m0), (m0, m1)
     assert_bitwise(U2, U1, "margin rerun")
+
+
+def test_margin_source_gradient_matches_finite_differences():
+    """The margin's source-inversion gradient at 48x24: the gauge misfit
+    0.5 |G(eta0) - G_obs|^2 over 40 adaptive steps (walls in x, periodic in
+    y, 12 gauges inside the uplift) under 4 snapshots and 4 slots, the same
+    bits as a tape holding every step, and within 1e-6 of fourth-order
+    central differences (eps 3e-3) of the plain solver's replay of the taped
+    steps in 6 seeded smooth directions (Gaussians 10-30 km wide)."""
+    import sgnx
+    from sgnx import cases
+    c = cases.margin(seed=3, Nx=48, Ny=24)
+    nd, steps = c.nd, 40
+    Hn = c.depth["Hnod"]
+    eta0, rest = c.U0[:nd] - Hn, c.U0[nd:]
+    nodes = cases.margin_gauges(c, 12, seed=3)
+    assert nodes.size == 12
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    s.upload(c.U0)
+    dt0 = s.cflTimeStep(None, 0.5)
+    opts = sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=steps)
+    s.upload(np.concatenate([Hn + 1.1 * eta0, rest]))
+    s.setGauges(nodes, steps)
+    s.integrateAdaptive(0.0, 1e9, dt0, opts)
+    G_obs = s.gaugeLog()[1]
+    grads = []
+    for snaps, slots in ((1 << 20, -1), (4, 4)):
+        s.upload(c.U0)
+        s.setGauges(nodes, steps)
+        s.tapeBegin(snaps)
+        st = s.integrateAdaptive(0.0, 1e9, dt0, opts)
+        s.tapeEnd()
+        s.tapeSlots(slots)
+        assert st.accepted == steps
+        G = s.gaugeLog()[1]
+        grads.append(s.adjoint(None, G - G_obs, theta=False)[:nd])
+    assert s.tapeInfo()[2] <= 4
+    assert_bitwise(grads[1], grads[0], "margin gradient: 4 snapshots + 4 slots vs every step")
+    g_eta = grads[1]
+
+    def misfit(e):
+        s.upload(np.concatenate([Hn + e, rest]))
+        s.setGauges(nodes, steps)
+        s.tapeReplay()
+        return 0.5 * np.sum((s.gaugeLog()[1] - G_obs) ** 2)
+
+    rng = np.random.default_rng(11)
+    eps, worst = 3e-3, 0.0
+    for _ in range(6):
+        de = cases.smooth_direction(c, rng, n=2, near=nodes, scale=5000.0)
+        f = {k: misfit(eta0 + k * eps * de) for k in (-2, -1, 1, 2)}
+        fd = (8.0 * (f[1] - f[-1]) - (f[2] - f[-2])) / (12.0 * eps)
+        pe = g_eta @ de
+        worst = max(worst, abs(pe - fd) / max(abs(pe), abs(fd)))
+    s.close()
+    assert worst < 1e-6, worst
