@@ -40,8 +40,10 @@
 namespace sgnx {
 namespace {
 
-constexpr int kT = 256;    // threads per block
-constexpr int kEW = 10;    // elements per block of k_vjp_elem (one element row)
+constexpr int kT = 256;    // threads per block of the elementwise kernels
+constexpr int kEW = 5;     // elements per tile of k_vjp_coef / k_vjp_elem (one element row)
+constexpr int kVT = ((kEW * NPP + 63) / 64) * 64;   // their threads per block (whole waves)
+static_assert(20 * kEW <= kVT, "one thread per element face node");
 
 // --- pointwise adjoints of sgnx_phys.h (nManning = 0) --------------------------
 // Every term is differentiated with the primitives u = hu/h, v, w, p as
@@ -537,12 +539,12 @@ __device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry&
   }
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kVT)
 k_vjp_coef(VjpArgs va, Geometry g, Phys ph) {
   vjp_coef_tile(va, g, ph, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kVT)
 k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
   vjp_elem_tile(va, g, ph, gt, blockIdx.x);
 }
@@ -559,7 +561,7 @@ __device__ __forceinline__ int member_of(const long long* tileStart, int nMem, l
   return lo;
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kVT)
 k_vjp_coef_batch(const VjpMember* __restrict__ mem, const long long* __restrict__ tileStart,
                  int nMem) {
   const long long w = blockIdx.x;
@@ -567,7 +569,7 @@ k_vjp_coef_batch(const VjpMember* __restrict__ mem, const long long* __restrict_
   vjp_coef_tile(mem[m].va, mem[m].g, mem[m].ph, (int)(w - tileStart[m]));
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kVT)
 k_vjp_elem_batch(const VjpMember* __restrict__ mem, const long long* __restrict__ tileStart,
                  int nMem) {
   const long long w = blockIdx.x;
@@ -653,11 +655,11 @@ hipError_t launch_rhs_vjp(const Geometry& g, const Phys& ph, const VjpArgs& va, 
                           hipStream_t st) {
   const long long nb = (long long)g.Ny * ((g.Nx + kEW - 1) / kEW);
   if (!va.coefReady) {
-    hipLaunchKernelGGL(k_vjp_coef, dim3((unsigned)nb), dim3(kT), 0, st, va, g, ph);
+    hipLaunchKernelGGL(k_vjp_coef, dim3((unsigned)nb), dim3(kVT), 0, st, va, g, ph);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_vjp_elem, dim3((unsigned)nb), dim3(kT), 0, st, va, g, ph, gt);
+  hipLaunchKernelGGL(k_vjp_elem, dim3((unsigned)nb), dim3(kVT), 0, st, va, g, ph, gt);
   return hipGetLastError();
 }
 
@@ -669,12 +671,12 @@ hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart
                                 long long totalTiles, bool coefReady, hipStream_t st) {
   if (totalTiles <= 0) return hipSuccess;
   if (!coefReady) {
-    hipLaunchKernelGGL(k_vjp_coef_batch, dim3((unsigned)totalTiles), dim3(kT), 0, st, mem,
+    hipLaunchKernelGGL(k_vjp_coef_batch, dim3((unsigned)totalTiles), dim3(kVT), 0, st, mem,
                        tileStart, nMem);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_vjp_elem_batch, dim3((unsigned)totalTiles), dim3(kT), 0, st, mem,
+  hipLaunchKernelGGL(k_vjp_elem_batch, dim3((unsigned)totalTiles), dim3(kVT), 0, st, mem,
                      tileStart, nMem);
   return hipGetLastError();
 }
