@@ -208,10 +208,11 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
                          const sgnx_rk_opts* o, sgnx_rk_stats* st, int flags);
 
 /* sgnx_adjoint for every member, their reverse sweeps in lockstep with one
- * launch per kind over all members (aligned at each member's last step);
- * each member's cotangents are bit for bit its own sgnx_adjoint's. Arrays of
- * n pointers (Ubar_final, gauge_bar, theta_bar may be NULL, or hold NULLs);
- * every member's tape must hold every step's snapshot. */
+ * launch per kind over all members (aligned at each member's last step), or,
+ * when every member alone fills the GPU, the members' own sweeps back to
+ * back; each member's cotangents are bit for bit its own sgnx_adjoint's.
+ * Arrays of n pointers (Ubar_final, gauge_bar, theta_bar may be NULL, or hold
+ * NULLs); every member's tape must hold every step's snapshot. */
 int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
                        const double* const* gauge_bar, double* const* U0bar,
                        double* const* theta_bar, int mem);

@@ -500,7 +500,11 @@ int sgnx_batch_autotune(sgnx_batch* b, int reps, int* rows_out, int* waves3_out)
 // RHS recomputations, the two stage combinations, the three products), each
 // member on its own buffers. The arithmetic is that of sgnx_adjoint on each
 // member alone, so the cotangents are bit for bit the same. Every member's
-// tape must hold every step's snapshot.
+// tape must hold every step's snapshot. When every member alone gives each
+// compute unit a full round of product workgroups (6 of 128 work-items), the
+// members' own sweeps run back to back instead: the products are bound by
+// their traffic, so the lockstep's launches gain nothing there, and its
+// per-step argument tables and member lookups made it slower (same bits).
 int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
                        const double* const* gauge_bar, double* const* U0bar,
                        double* const* theta_bar, int mem) {
@@ -509,6 +513,24 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
     return bfail(b, SGNX_EINVAL, "batch adjoint: bad memory kind");
   const int n = b->n;
   BCHK(b, hipSetDevice(b->device));
+  int cus = 0;
+  BCHK(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  bool each_fills = n > 0;
+  for (int i = 0; i < n; ++i) each_fills = each_fills && vjp_tiles(b->m[i]->g) >= 6LL * cus;
+  if (each_fills) {
+    for (int i = 0; i < n; ++i) {
+      sgnx_solver* s = b->m[i];
+      if (!U0bar[i]) return bfail(b, SGNX_EINVAL, "batch adjoint: U0bar missing for a member");
+      AdjTape* t = nullptr;
+      int rc = adjoint_begin(s, true, &t);
+      if (!rc)
+        rc = sgnx_adjoint(s, Ubar_final ? Ubar_final[i] : nullptr,
+                          gauge_bar ? gauge_bar[i] : nullptr, U0bar[i],
+                          theta_bar ? theta_bar[i] : nullptr, mem);
+      if (rc) return bfail(b, rc, "batch adjoint, member " + std::to_string(i) + ": " + s->err);
+    }
+    return SGNX_OK;
+  }
   std::vector<AdjTape*> tp(n, nullptr);
   std::vector<const double*> gb(n, nullptr);
   long Nmax = 0;

@@ -614,3 +614,39 @@ def test_batched_forward_records_the_solo_tapes(keep):
     b.close()
     for s in ss:
         s.close()
+
+
+def test_batched_adjoint_of_large_members_matches_solo_bitwise():
+    """Members that each fill the GPU (at least six product workgroups per
+    compute unit) run their own sweeps inside the batched call: every
+    member's cotangents are its own sgnx_adjoint's bit for bit, and a
+    thinned tape is refused as in the lockstep."""
+    import sgnx
+    from sgnx import cases
+    cs = [cases.shoal(seed=5, Nx=160, Ny=80), cases.shoal(seed=6, Nx=170, Ny=86)]
+    steps = [4, 3]
+    rng = np.random.default_rng(78)
+    ss, cot, solo = [], [], []
+    for c, k in zip(cs, steps):
+        nodes = rng.choice(c.nd, size=4, replace=False).astype(np.int64)
+        s, UN, G = _taped_run(c, c.depth["Hnod"], c.U0, nodes, k, 1 << 20)
+        ub, gb = rng.normal(size=UN.size), rng.normal(size=G.shape)
+        solo.append(s.adjoint(ub, gb))
+        cot.append((ub, gb))
+        ss.append(s)
+    b = sgnx.Batch(ss)
+    res = b.adjoint([u for u, _ in cot], [g for _, g in cot])
+    for i, ((U0b, thb), (sU, sT)) in enumerate(zip(res, solo)):
+        assert_bitwise(U0b, sU, f"large member {i} U0bar")
+        assert_bitwise(thb, sT, f"large member {i} thetaBar")
+    c = cs[0]
+    ss[0].upload(c.U0)
+    ss[0].tapeBegin(2)
+    ss[0].integrateAdaptive(0.0, 1e9, ss[0].cflTimeStep(None, 0.5),
+                            sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=6))
+    ss[0].tapeEnd()
+    with pytest.raises(sgnx.SgnxError, match="every step"):
+        b.adjoint()
+    b.close()
+    for s in ss:
+        s.close()
