@@ -90,8 +90,9 @@ class Batch:
 
     def adjoint(self, Ubar_final=None, gauge_bar=None, theta=True):
         """Every member's SGN2D.adjoint (its tape recorded with a snapshot of
-        every step), the reverse sweeps in lockstep with one launch per kind;
-        each member's result is bit for bit its own adjoint's. Ubar_final,
+        every step), the reverse sweeps in lockstep with one launch per kind
+        (or, when every member alone fills the GPU, the members' own sweeps
+        back to back); each member's result is bit for bit its own adjoint's. Ubar_final,
         gauge_bar: None or one array (or None) per member. Returns a list of
         (U0bar, thetaBar) (or U0bar when theta is False)."""
         n = len(self.members)
