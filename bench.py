@@ -16,15 +16,14 @@ synchronisations (state resident in HBM before the clock starts).
 
 Also reported:
   roofline      the RHS kernel (k_rhs_sweep) as the timed run launches it, with
-                the RK stage fused: the algorithmic bytes of each launch kind
-                (ALG_BYTES_BY_KIND: the RHS's 104 B per node -- U read, dU
-                written, H, Hx, Hy read -- plus the stage operands that launch
-                reads and writes), averaged over the run's launches, over their
+                the RK stage fused: the RHS's algorithmic 104 B per node (U
+                read, dU written, H, Hx, Hy read) per launch over the launches'
                 mean duration, measured with HIP events around every launch,
-                against the 8 TB/s HBM3E peak. Beside it: the same launches
-                priced at the RHS's 104 B per node only, and the plain RHS on
-                its own (20 launches after the timed run);
-                `traffic` is the
+                against the 8 TB/s HBM3E peak (`achieved`, `frac`). Beside it:
+                the same launches priced with the stage operands each launch
+                kind also reads and writes (ALG_BYTES_BY_KIND,
+                `achieved_stage_bytes`), and the plain RHS on its own (20
+                launches after the timed run); `traffic` is the
                 measured HBM bytes per launch from the rocprofv3 counter runs
                 summarised in profiles/rhs_traffic.json (per launch kind,
                 weighted by the timed run's launch mix; null if absent)
@@ -172,7 +171,7 @@ def plain_rhs_rate(s, nd, n=20):
             "frac": gbs / HBM_PEAK_GBS}
 
 
-def batched_sweep(args, nx_min=128, nx_max=768, n_members=None, steps=None):
+def batched_sweep(args, nx_min=128, nx_max=768, n_members=None, steps=None, single_rate=None):
     """The batched shoal sweep: --batch-members grids of seeded uneven sizes
     (Nx in [128, 768], Ny = Nx/2; cases.sweep_member), each advanced
     --batch-steps accepted adaptive steps, first one at a time with the plain
@@ -180,7 +179,11 @@ def batched_sweep(args, nx_min=128, nx_max=768, n_members=None, steps=None):
     captured step graph); every member must end bit for bit where its solo run
     did. The tile height is chosen once by the batch autotuner and used by
     both runs. Before that, the members also run one at a time with their
-    own default tile heights: the plain solver's rate without the batch."""
+    own default tile heights: the plain solver's rate without the batch,
+    against which `speedup` is taken (each member's best solo setting).
+    `speedup_bound` is what batching could gain at most if every member ran
+    at `single_rate` (the rate of one production-size grid): the members'
+    solo time over their work at that rate."""
     import hashlib
     import sgnx
     from sgnx import cases
@@ -245,14 +248,19 @@ def batched_sweep(args, nx_min=128, nx_max=768, n_members=None, steps=None):
            "sequential": {"value": seq_dof / seq_wall, "wall_s": seq_wall},
            "sequential_own_tiles": {"value": own_dof / own_wall, "wall_s": own_wall},
            "batched": {"value": bat_dof / bat_wall, "wall_s": bat_wall, "graph": True},
-           "speedup": (bat_dof / bat_wall) / (seq_dof / seq_wall),
-           "speedup_vs_own_tiles": (bat_dof / bat_wall) / (own_dof / own_wall),
+           "speedup": (bat_dof / bat_wall) / (own_dof / own_wall),
+           "speedup_basis": "each member's own default tile height (its best solo setting)",
+           "speedup_vs_batch_tiles": (bat_dof / bat_wall) / (seq_dof / seq_wall),
            "members_bit_identical": bool(same), "unit": "DOF-RHS-evals/s"}
+    if single_rate:
+        out["single_grid_rate"] = single_rate
+        out["speedup_bound"] = own_wall / (own_dof / single_rate)
     log(f"[bench] batch: sequential {seq_dof / seq_wall / 1e9:.2f} GDOF-RHS/s ({seq_wall:.2f}s), "
         f"batched {bat_dof / bat_wall / 1e9:.2f} GDOF-RHS/s ({bat_wall:.2f}s), "
-        f"x{out['speedup']:.2f}, identical={same}, rows {rows} waves {waves:#x}; "
+        f"x{out['speedup_vs_batch_tiles']:.2f}, identical={same}, rows {rows} waves {waves:#x}; "
         f"one at a time with each grid's own tile height {own_dof / own_wall / 1e9:.2f} "
-        f"GDOF-RHS/s (x{out['speedup_vs_own_tiles']:.2f})")
+        f"GDOF-RHS/s: speed-up x{out['speedup']:.2f}"
+        + (f" (bound x{out['speedup_bound']:.2f} at the single-grid rate)" if single_rate else ""))
     return out
 
 
@@ -747,13 +755,52 @@ def cpu_baseline(case, gpu_solver, dt0, args):
             "aggregate": agg}
 
 
+def host_cpus():
+    """(nproc, the cgroup's CPU quota in whole CPUs or None): the cores this
+    process may run on, and the share a CPU controller grants them."""
+    nproc = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(-(-int(q) // int(per))))
+    except (OSError, ValueError):
+        pass
+    return nproc, quota
+
+
+def oracle_instance_bytes(nd):
+    """Host bytes one oracle instance holds at nd nodes: the state, its RHS and
+    the solver's arrays (nodal depths, face values and fluxes, scratch), about
+    40 doubles per node."""
+    return 40 * 8 * nd
+
+
 def cpu_aggregate(case, OL, reps=2):
-    """N independent single-threaded oracle instances at once (N = the host
-    cores this process may use, at most 16), each evaluating the full-size RHS
-    `reps` times: the host's aggregate rate. ctypes releases the interpreter
-    lock during the oracle's calls, so the threads run in parallel."""
+    """N independent single-threaded oracle instances at once, each evaluating
+    the full-size RHS `reps` times: the host's aggregate rate. N = nproc (the
+    cores this process may use), lowered only where the cgroup's CPU quota or
+    the host memory available cannot hold that many instances; both figures
+    are reported. ctypes releases the interpreter lock during the oracle's
+    calls, so the threads run in parallel."""
     import threading
-    n = min(16, len(os.sched_getaffinity(0)))
+    nproc, quota = host_cpus()
+    n = nproc
+    if quota:
+        n = min(n, quota)
+    avail = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemAvailable:"):
+                    avail = int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    mem_cap = None
+    if avail:
+        mem_cap = max(1, int(0.5 * avail // oracle_instance_bytes(case.nd)))
+        n = min(n, mem_cap)
     orcs = [OL.Oracle.from_case(case) for _ in range(n)]
     start = threading.Barrier(n + 1)
 
@@ -776,7 +823,10 @@ def cpu_aggregate(case, OL, reps=2):
     log(f"[bench] cpu aggregate: {n} instances x {reps} RHS in {wall:.2f}s "
         f"({value / 1e6:.1f} MDOF-RHS/s)")
     return {"value": value, "unit": "DOF-RHS-evals/s", "cores": n, "instances": n,
-            "sample": f"{n} concurrent single-threaded oracle instances, {reps} full-size RHS each"}
+            "nproc": nproc, "cgroup_cpu_quota": quota, "memory_cap_instances": mem_cap,
+            "sample": (f"{n} concurrent single-threaded oracle instances (nproc {nproc}, "
+                       f"cgroup CPU quota {quota}, memory room for {mem_cap}), "
+                       f"{reps} full-size RHS each")}
 
 
 def main():
@@ -893,7 +943,7 @@ def main():
     batched = None
     if args.batch_members > 0:
         try:
-            batched = batched_sweep(args)
+            batched = batched_sweep(args, single_rate=value)
         except Exception as e:                    # reported, not fatal to the main line
             batched = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] batch leg failed: {batched['error']}")
@@ -903,7 +953,7 @@ def main():
         try:
             batched_small = batched_sweep(args, nx_min=32, nx_max=128,
                                           n_members=args.batch_small_members,
-                                          steps=args.batch_small_steps)
+                                          steps=args.batch_small_steps, single_rate=value)
         except Exception as e:                    # reported, not fatal to the main line
             batched_small = {"error": f"{type(e).__name__}: {e}"}
             log(f"[bench] small-grid batch leg failed: {batched_small['error']}")
@@ -973,18 +1023,23 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved_stage,
+            "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved_stage / HBM_PEAK_GBS,
+            "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "kernel": "k_rhs_sweep (FR/SGN right-hand side with the RK stage fused)",
-            "alg_bytes_per_launch": alg_stage_per_launch,
-            "alg_bytes_per_node_mean": alg_stage_per_launch / nd,
+            "basis": ("104 algorithmic bytes per node per RHS launch (U read, dU written, "
+                      "H, Hx, Hy read) over the launches' mean duration"),
+            "alg_bytes_per_launch": ALG_BYTES_PER_NODE * nd,
             "mean_launch_ms": mean_rhs_s * 1e3,
             "launches": launches,
             "achieved_rhs_bytes_only": achieved,
             "frac_rhs_bytes_only": achieved / HBM_PEAK_GBS,
+            "achieved_stage_bytes": achieved_stage,
+            "frac_stage_bytes": achieved_stage / HBM_PEAK_GBS,
+            "alg_stage_bytes_per_launch": alg_stage_per_launch,
+            "alg_stage_bytes_per_node_mean": alg_stage_per_launch / nd,
             "plain_rhs": plain,
             "launches_by_kind": {k: timing[m][0] for m, k in enumerate(KIND_NAMES)},
             "mean_ms_by_kind": {k: (timing[m][1] / timing[m][0] if timing[m][0] else None)

@@ -49,6 +49,11 @@ struct sgnx_batch {
   hipGraphExec_t exec = nullptr;
   std::vector<int> graphSet;
   bool graphKeep = false;   // the captured step keeps stage inputs
+  // the launch shape baked into the captured step: its grid (the members'
+  // tiles in all) and each running member's tile height, so that a change of
+  // either (sgnx_set_rhs_kernel or sgnx_autotune on a member) forces a capture
+  long long graphTotal = 0;
+  std::vector<int> graphRows;
   // argument arena of the batched reverse sweep (pinned host, device)
   char* h_adj = nullptr;
   char* d_adj = nullptr;
@@ -80,6 +85,8 @@ void drop_graph(sgnx_batch* b) {
   b->exec = nullptr;
   b->graph = nullptr;
   b->graphSet.clear();
+  b->graphRows.clear();
+  b->graphTotal = 0;
 }
 
 BatchMember member_args(const sgnx_solver* s, const double* U, double* dU, const Stage& sa,
@@ -300,7 +307,10 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
     }
     const long long total = fill_offsets(b, set);
     if (useGraph) {
-      if (!b->exec || b->graphSet != set || b->graphKeep != keep) {
+      std::vector<int> rowsNow(A);
+      for (int slot = 0; slot < A; ++slot) rowsNow[slot] = b->m[set[slot]]->g.rows;
+      if (!b->exec || b->graphSet != set || b->graphKeep != keep || b->graphTotal != total ||
+          b->graphRows != rowsNow) {
         drop_graph(b);
         BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         rc = enqueue_step(b, set, total, keep);
@@ -315,6 +325,8 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
         BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->graphSet = set;
         b->graphKeep = keep;
+        b->graphTotal = total;
+        b->graphRows = rowsNow;
       }
       BCHK(b, hipGraphLaunch(b->exec, b->stream));
     } else {

@@ -842,8 +842,10 @@ k_rhs_sweep_batch(const BatchMember* __restrict__ mem, const long long* __restri
     if (tileStart[mid] <= w) lo = mid; else hi = mid - 1;
   }
   const BatchMember& M = mem[lo];
-  sweep_tile<TW, MODE, WAVES, KEEP>(M.U, M.dU, M.g, M.ph, M.gt, M.sa, M.nStrips,
-                                    (int)(w - tileStart[lo]));
+  const long long tile = w - tileStart[lo];
+  // a grid larger than the members' tiles (never launched so) does nothing
+  if (tile >= M.nTiles) return;
+  sweep_tile<TW, MODE, WAVES, KEEP>(M.U, M.dU, M.g, M.ph, M.gt, M.sa, M.nStrips, (int)tile);
 }
 
 // Hx, Hy at every node (sgn2d.cpp:54-71).

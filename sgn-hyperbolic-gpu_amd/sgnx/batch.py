@@ -112,7 +112,8 @@ class Batch:
         U0b = [np.empty(m.nTotal()) for m in self.members]
         thb = [np.empty(m.nDof()) for m in self.members] if theta else None
         uf = ptrs(Ubar_final, [m.nTotal() for m in self.members])
-        gb = ptrs(gauge_bar, None)
+        gb = ptrs(gauge_bar, [m.gaugeValues() for m in self.members]
+                  if gauge_bar is not None else None)
         ub = ptrs(U0b, None)
         tb = ptrs(thb, None) if theta else None
         self._check(self.lib.sgnx_batch_adjoint(

@@ -23,6 +23,11 @@ Workloads
              across a seeded shelf break with 2 % seeded roughness, walls in x,
              periodic in y, a seeded 0.5-2 m elliptical Gaussian uplift at rest
              (the large single-grid case)
+  shoal_focus  384x192 on 40x20 m, periodic: a 0.3 m solitary wave over 1 m
+             crossing a Gaussian submerged shoal (the reference's production
+             focusing run, apps/shoal_focus_mpi.cu:88-137)
+  mms        the manufactured periodic solution of apps/convergence2d.cpp
+             (flat bottom, unit square): its state and forcing
 """
 from __future__ import annotations
 
@@ -188,6 +193,97 @@ def inflow_table(Ny, P, H_left, period, amp, g=G, gamma=GAMMA, alpha=ALPHA):
         tab[i, 3] = (0.0, 0.0, -gamma * Pamp * amp / omega)
         tab[i, 4] = (0.0, H * Pamp * amp, 0.0)
     return tab.reshape(-1), omega
+
+
+def shoal_focus(Nx=384, Ny=192, P=4, d_shoal=0.65, r_shoal=3.0):
+    """Dispersive focusing over a submerged shoal (apps/shoal_focus_mpi.cu:
+    98-137): still depth h0 - d exp(-((x - 18)^2 + (y - 10)^2) / (2 r^2)) on
+    a periodic 40x20 m domain, a solitary wave of amplitude a = 0.3 over
+    h0 = 1 m with its crest at x = 6 m riding on that depth, c = 3 sqrt(g h0).
+    meta: the shoal centre, the amplitude, the end time 8 s and the 100
+    samples of the reference's run."""
+    Lx, Ly = 40.0, 20.0
+    h0, a = 1.0, 0.3
+    xs, ys = 18.0, 10.0
+    par = Params(g=G, c=ALPHA * math.sqrt(G * h0), gamma=GAMMA)
+
+    def depth_fn(x, y):
+        dx, dy = x - xs, y - ys
+        return h0 - d_shoal * np.exp(-(dx * dx + dy * dy) / (2.0 * r_shoal * r_shoal))
+
+    bc = ("periodic",) * 4
+    depth = depth_arrays(P, Nx, Ny, Lx, Ly, depth_fn, bc)
+    x, _ = node_coords(P, Nx, Ny, Lx, Ly)
+    h, u, w, p = solitary_state(x, depth["Hnod"], h0, a, G, GAMMA, x0=6.0)
+    U0 = prim_to_state(h, u, np.zeros_like(h), w, p)
+    return Case("shoal_focus", P, Nx, Ny, Lx, Ly, par, depth, bc, U0,
+                meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "a": a, "x_shoal": xs,
+                      "y_shoal": ys, "t_end": 8.0, "samples": 100, "d_shoal": d_shoal,
+                      "r_shoal": r_shoal})
+
+
+# manufactured solution of apps/convergence2d.cpp:38-84 (phi = kx x + ky y - w t)
+MMS = {"g": 9.81, "gamma": 1.5, "alpha": 3.0, "L": 1.0, "k": 2.0 * math.pi, "omega": 1.0,
+       "ah": 0.10, "au": 0.15, "av": 0.12, "aw": 0.05, "ap": 0.03}
+
+
+def mms_prims(x, y, t):
+    """(h, u, v, w, p) of the manufactured solution and their x, y and t
+    derivatives, as convergence2d.cpp:43-52 forms them."""
+    m = MMS
+    kx = ky = m["k"]
+    om = m["omega"]
+    ah, au, av, aw, ap = m["ah"], m["au"], m["av"], m["aw"], m["ap"]
+    ph = kx * x + ky * y - om * t
+    C, S = np.cos(ph), np.sin(ph)
+    val = (1.0 + ah * C, au * C, av * C, aw * S, ap * C)
+    dx = (-ah * kx * S, -au * kx * S, -av * kx * S, aw * kx * C, -ap * kx * S)
+    dy = (-ah * ky * S, -au * ky * S, -av * ky * S, aw * ky * C, -ap * ky * S)
+    dt = (ah * om * S, au * om * S, av * om * S, -aw * om * C, ap * om * S)
+    return val, dx, dy, dt
+
+
+def mms_forcing(x, y, t, c2):
+    """G = d_t U* - R_cont(U*) of the manufactured solution, the five rows of
+    convergence2d.cpp:54-84 term by term (shape (5, n))."""
+    g, gamma = MMS["g"], MMS["gamma"]
+    (h, u, v, w, p), (hx, ux, vx, wx, px), (hy, uy, vy, wy, py), (ht, ut, vt, wt, pt) = \
+        mms_prims(x, y, t)
+    dtU = (ht, ht * u + h * ut, ht * v + h * vt, ht * w + h * wt, ht * p + h * pt)
+    dxF1 = hx * u + h * ux
+    dxF2 = hx * u * u + 2.0 * h * u * ux + g * h * hx + hx * p + h * px
+    dxF3 = hx * u * v + h * ux * v + h * u * vx
+    dxF4 = hx * u * w + h * ux * w + h * u * wx
+    dxF5 = (hx * u + h * ux) * (p + c2) + h * u * px
+    dyG1 = hy * v + h * vy
+    dyG2 = hy * u * v + h * uy * v + h * u * vy
+    dyG3 = hy * v * v + 2.0 * h * v * vy + g * h * hy + hy * p + h * py
+    dyG4 = hy * v * w + h * vy * w + h * v * wy
+    dyG5 = (hy * v + h * vy) * (p + c2) + h * v * py
+    R = (-(dxF1 + dyG1), -(dxF2 + dyG2), -(dxF3 + dyG3), -(dxF4 + dyG4) + gamma * p,
+         -(dxF5 + dyG5) + c2 * u * hx + c2 * v * hy - 2.0 * c2 * w)
+    return np.stack([dtU[k] - R[k] for k in range(5)])
+
+
+def mms(N, P=4):
+    """The manufactured-solution case at N x N elements on the unit square,
+    flat bottom (H = 0), periodic; meta: the RK4 step of convergence2d.cpp:
+    116-122 (dt = 0.15 dx_eff / sqrt(1.2 g + c^2), rounded to land on t = 0.1)."""
+    m = MMS
+    L = m["L"]
+    c = m["alpha"] * math.sqrt(m["g"] * 1.0)
+    par = Params(g=m["g"], c=c, gamma=m["gamma"])
+    x, y = node_coords(P, N, N, L, L)
+    (h, u, v, w, p), _, _, _ = mms_prims(x, y, 0.0)
+    U0 = prim_to_state(h, u, v, w, p)
+    t_end = 0.1
+    dxeff = (L / N) / (2.0 * P + 1.0)
+    sig = math.sqrt(m["g"] * 1.2 + c * c)
+    dt = 0.15 * dxeff / sig
+    n_step = int(math.ceil(t_end / dt))
+    dt = t_end / n_step
+    return Case(f"mms{N}", P, N, N, L, L, par, None, ("periodic",) * 4, U0,
+                meta={"t_end": t_end, "dt": dt, "steps": n_step, "c2": c * c, "x": x, "y": y})
 
 
 def shoal_params(seed):

@@ -304,6 +304,11 @@ class SGN2D:
         self._nG = int(nodes.size)
         self._gcap = int(cap)
 
+    def gaugeValues(self):
+        """Number of values in the gauge log (rows recorded x gauges): the size
+        a gauge-log cotangent must have."""
+        return self.gaugeLog()[1].size
+
     def gaugeLog(self):
         """(times, eta[rows, n]) recorded since setGauges."""
         n, cap = getattr(self, "_nG", 0), getattr(self, "_gcap", 0)
@@ -413,6 +418,10 @@ class SGN2D:
         gb = None
         if gauge_bar is not None:
             gb = np.ascontiguousarray(gauge_bar, dtype=np.float64)
+            want = self.gaugeValues()
+            if gb.size != want:
+                # the library reads rows x gauges values from this buffer
+                raise ValueError(f"gauge_bar has {gb.size} values, the gauge log {want}")
         U0b = np.empty(self.nTotal())
         tb = np.empty(self._Nd) if theta else None
         L.check(self.lib.sgnx_adjoint(self.h, Uf.ctypes.data if Uf is not None else None,
