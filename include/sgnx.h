@@ -172,6 +172,16 @@ int sgnx_eta_max(sgnx_solver* s, double x0, double x1, double y0, double y1, dou
                  long long* node);
 /* eta = h - H at n nodes (the gauges of apps/tsunami_margin_mpi.cu:274-278). */
 int sgnx_gauges(sgnx_solver* s, const long long* nodes, int n, double* eta);
+/* sgnx_gauges without allocating: no counterpart in the reference. For callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc). The node list and the values pass through a static device array
+ * of the library (64 KiB per GPU, held until the process ends), so at most
+ * SGNX_GAUGES_NOALLOC_MAX gauges per call (more: SGNX_EINVAL). Every call on
+ * one GPU shares that array: calls that overlap on one GPU (several host
+ * threads, or solvers on different streams) race. Same values as
+ * sgnx_gauges, bit for bit. */
+#define SGNX_GAUGES_NOALLOC_MAX 4096
+int sgnx_gauges_noalloc(sgnx_solver* s, const long long* nodes, int n, double* eta);
 /* Element means of eta, u, v, p, H as src/io/vtk2d.h:39-81 forms them (0.25 *
  * sum over b, a of qw[a] qw[b] f): out[f * Nx * Ny + ey * Nx + ex], 5 fields. */
 int sgnx_element_means(sgnx_solver* s, double* out);

@@ -20,17 +20,45 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "sgnx_handle.h"
 
+// Scratch of sgnx_gauges_noalloc: the node list (SGNX_GAUGES_NOALLOC_MAX
+// 64-bit indices) followed by the gathered values (as many doubles), 64 KiB.
+// Every GPU the library runs on holds its own copy until the process ends, and
+// every call on that GPU shares it.
+static __device__ __align__(256) unsigned char
+    scratch[SGNX_GAUGES_NOALLOC_MAX * (sizeof(long long) + sizeof(double))];
+
 
 
 namespace {
 
 thread_local std::string g_create_error;
+
+// Device address of `scratch` on each GPU, looked up once per device with that
+// device current. In host code the array's name is only a stand-in for the
+// symbol: kernels and copies must be given this address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_address(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 int fail(sgnx_solver* s, int code, const std::string& msg) {
   if (s) s->err = msg;
@@ -176,6 +204,7 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   } while (0)
 
   CK(hipGetDevice(&s->device));
+  CK(scratch_address(s->device, &s->scratch));
   if (!std::getenv("SGNX_SWEEP_ROWS")) {
     // default tile height: the tallest that still gives every compute unit
     // about three workgroups (small grids would leave most of the chip idle
@@ -455,6 +484,26 @@ int sgnx_gauges(sgnx_solver* s, const long long* nodes, int n, double* eta) {
   (void)hipFree(dn);
   if (dv) (void)hipFree(dv);
   HIPCHK(s, e);
+  return SGNX_OK;
+}
+
+int sgnx_gauges_noalloc(sgnx_solver* s, const long long* nodes, int n, double* eta) {
+  if (!s || (n > 0 && (!nodes || !eta)) || n < 0) return SGNX_EINVAL;
+  if (n > SGNX_GAUGES_NOALLOC_MAX)
+    return fail(s, SGNX_EINVAL, "gauges_noalloc: " + std::to_string(n) + " gauges, at most " +
+                                    std::to_string(SGNX_GAUGES_NOALLOC_MAX) + " per call");
+  if (n == 0) return SGNX_OK;
+  for (int k = 0; k < n; ++k)
+    if (nodes[k] < 0 || nodes[k] >= s->g.Nd) return fail(s, SGNX_EINVAL, "gauges: node out of range");
+  if (!s->scratch) return fail(s, SGNX_EDEVICE, "gauges_noalloc: no scratch address");
+  // the node list, then the values: every entry read is written first by
+  // this call, so nothing needs clearing
+  long long* dn = static_cast<long long*>(s->scratch);
+  double* dv = reinterpret_cast<double*>(dn + SGNX_GAUGES_NOALLOC_MAX);
+  HIPCHK(s, hipMemcpyAsync(dn, nodes, n * sizeof(long long), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(s, launch_gather_eta(s->g, s->d_U, dn, n, dv, s->stream));
+  HIPCHK(s, hipMemcpyAsync(eta, dv, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
   return SGNX_OK;
 }
 

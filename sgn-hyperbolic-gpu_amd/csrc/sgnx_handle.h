@@ -50,6 +50,9 @@ struct sgnx_solver {
   std::vector<long long> gnodes;    // host copy of the gauge nodes
   // adjoint tape (accepted steps and state snapshots of the recorded runs)
   struct AdjTape* tape = nullptr;
+  // this GPU's copy of the static scratch array of sgnx_gauges_noalloc (its
+  // device address, looked up once per device by sgnx_create)
+  void* scratch = nullptr;
 
   size_t n() const { return (size_t)NV * (size_t)g.Nd; }
 };

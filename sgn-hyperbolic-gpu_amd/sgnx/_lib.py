@@ -100,6 +100,7 @@ SIGNATURES = [
     ("sgnx_eta_max", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _dp,
                                C.POINTER(C.c_longlong)]),
     ("sgnx_gauges", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int, _dp]),
+    ("sgnx_gauges_noalloc", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int, _dp]),
     ("sgnx_element_means", C.c_int, [C.c_void_p, _dp]),
     ("sgnx_set_gauges", C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.c_int, C.c_long]),
     ("sgnx_gauge_log", C.c_int, [C.c_void_p, _dp, _dp, C.c_long, C.POINTER(C.c_long)]),

@@ -281,12 +281,14 @@ class SGN2D:
                                       C.byref(v), C.byref(i)), self.h)
         return v.value, i.value
 
-    def gauges(self, nodes):
-        """eta at the given node indices of the resident state."""
+    def gauges(self, nodes, noalloc=False):
+        """eta at the given node indices of the resident state (noalloc: through
+        sgnx_gauges_noalloc, at most SGNX_GAUGES_NOALLOC_MAX nodes)."""
         nodes = np.ascontiguousarray(nodes, dtype=np.int64)
         out = np.zeros(nodes.size)
-        L.check(self.lib.sgnx_gauges(self.h, nodes.ctypes.data_as(C.POINTER(C.c_longlong)),
-                                     int(nodes.size), _dp(out)), self.h)
+        fn = self.lib.sgnx_gauges_noalloc if noalloc else self.lib.sgnx_gauges
+        L.check(fn(self.h, nodes.ctypes.data_as(C.POINTER(C.c_longlong)), int(nodes.size),
+                   _dp(out)), self.h)
         return out
 
     def elementMeans(self):
