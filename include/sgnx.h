@@ -146,7 +146,10 @@ int sgnx_bathymetry(const sgnx_solver* s, double* Hx, double* Hy);
  * element columns and `rows` element rows per workgroup (default 1, rows 16,
  * or fewer on grids too small to give every compute unit three workgroups;
  * the environment variables SGNX_RHS_VARIANT / SGNX_SWEEP_ROWS set the
- * defaults at creation). Both are bitwise equal to the reference rhs(). */
+ * defaults at creation). Both are bitwise equal to the reference rhs(). The
+ * row sweep has no Manning friction term: a solver with nManning != 0 uses
+ * variant 0 (variant 1 then gives SGNX_EUNSUPPORTED, and such a solver
+ * cannot join a batch or be autotuned). */
 int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows);
 
 /* Launch timing (instrumentation, no counterpart in the reference): while

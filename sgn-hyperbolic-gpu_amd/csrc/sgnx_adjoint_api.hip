@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <new>
 #include <string>
 #include <vector>
@@ -471,6 +472,9 @@ int sgnx_set_depth(sgnx_solver* s, const double* theta, size_t n, int mem) {
   HIPCHK(s, launch_bathy_grad(s->g, s->d_Hnod, s->d_Hx, s->d_Hy, s->stream));
   HIPCHK(s, launch_face_depth(s->g, s->d_Hnod, s->d_HfX, s->d_HfY, s->stream));
   HIPCHK(s, hipStreamSynchronize(s->stream));
+  // Hx, Hy are now the arithmetic the row sweep can repeat from Hnod
+  const char* v = std::getenv("SGNX_SWEEP_HXR");
+  s->g.hxr = !v || std::atoi(v) != 0;
   return SGNX_OK;
 }
 

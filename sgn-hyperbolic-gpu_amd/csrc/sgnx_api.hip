@@ -47,6 +47,11 @@ constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 void* g_scratch_ptr[kMaxDevices] = {};
 
+bool hxr_allowed() {
+  const char* v = std::getenv("SGNX_SWEEP_HXR");
+  return !v || std::atoi(v) != 0;
+}
+
 hipError_t scratch_address(int dev, void** out) {
   if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
   std::lock_guard<std::mutex> lock(g_scratch_mu);
@@ -187,6 +192,9 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   ph.m2c2 = -2.0 * ph.c2;
   s->hasInflow = d->bc[0] == SGNX_BC_INFLOW;
   s->inflowOmega = d->inflow_omega;
+  // the row sweep leaves out the Manning friction term: with friction the
+  // element-row tiles and their separate stage passes evaluate the RHS
+  if (ph.nManning != 0.0) g.variant = 0;
 
   auto cleanup_fail = [&](int code) {
     std::string msg = s->err;
@@ -270,6 +278,17 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   }
   CK(hipMalloc((void**)&s->d_out, 2 * sizeof(double)));
   CK(hipHostMalloc((void**)&s->h_out, 2 * sizeof(double), hipHostMallocDefault));
+  {
+    // whether the row sweep may form Hx, Hy from Hnod: they were computed
+    // here, or the caller's equal that arithmetic bit for bit
+    // (SGNX_SWEEP_HXR=0 makes it read them regardless)
+    int* bad = reinterpret_cast<int*>(s->d_out);
+    CK(hipMemset(bad, 0, sizeof(int)));
+    CK(launch_bathy_check(g, s->d_Hnod, s->d_Hx, s->d_Hy, bad, nullptr));
+    int hb = 1;
+    CK(hipMemcpy(&hb, bad, sizeof(int), hipMemcpyDeviceToHost));
+    g.hxr = hb == 0 && hxr_allowed();
+  }
 #undef CK
   *out = s;
   return SGNX_OK;
@@ -601,6 +620,8 @@ int sgnx_energy(sgnx_solver* s, double* out) {
 
 int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows) {
   if (!s || variant < 0 || variant > 1 || rows < 1) return SGNX_EINVAL;
+  if (variant == 1 && s->ph.nManning != 0.0)
+    return fail(s, SGNX_EUNSUPPORTED, "the row sweep has no Manning friction term (variant 0 has)");
   s->g.variant = variant;
   s->g.rows = rows;
   return SGNX_OK;

@@ -117,6 +117,15 @@ long long fill_offsets(sgnx_batch* b, const std::vector<int>& set) {
 // the device work of one attempted step for the members in `set` (slots
 // 0..A-1 of kinds 1..3 of h_mem filled by the caller); `keep`: stages 2 and 3
 // also write their inputs where a member's stage names a buffer for them
+// whether every member's Hx, Hy are (D Hnod) / J bit for bit, so that the
+// batched sweep may form them from Hnod (otherwise it reads them, which is
+// right for every member)
+bool batch_hxr(const sgnx_batch* b) {
+  for (const sgnx_solver* s : b->m)
+    if (!s->g.hxr) return false;
+  return true;
+}
+
 int enqueue_step(sgnx_batch* b, const std::vector<int>& set, long long total, bool keep) {
   const int A = (int)set.size();
   const size_t n = (size_t)b->n;
@@ -126,7 +135,7 @@ int enqueue_step(sgnx_batch* b, const std::vector<int>& set, long long total, bo
                          b->stream));
   for (int k = 1; k <= 3; ++k)
     BCHK(b, launch_rhs_stage_batch(b->d_mem + k * n, b->d_ts, A, total, k,
-                                   (b->waves3 >> k) & 1, b->stream, keep && k < 3));
+                                   (b->waves3 >> k) & 1, b->stream, keep && k < 3, batch_hxr(b)));
   BCHK(b, launch_finish_batch(b->d_mem + 3 * n, A, b->d_out, b->stream));
   BCHK(b, hipMemcpyAsync(b->h_out, b->d_out, 2 * (size_t)A * sizeof(double),
                          hipMemcpyDeviceToHost, b->stream));
@@ -164,7 +173,10 @@ int sgnx_batch_create(sgnx_solver* const* members, int n, sgnx_batch** out) {
     const sgnx_solver* s = members[i];
     if (!s) return bfail(nullptr, SGNX_EINVAL, "batch: null member");
     if (s->g.variant != 1)
-      return bfail(nullptr, SGNX_EINVAL, "batch: members must use the row-sweep kernel");
+      return bfail(nullptr, SGNX_EINVAL,
+                   s->ph.nManning != 0.0
+                       ? "batch: members with Manning friction are not supported (no row sweep)"
+                       : "batch: members must use the row-sweep kernel");
     if (dev < 0) dev = s->device;
     if (s->device != dev) return bfail(nullptr, SGNX_EINVAL, "batch: members on different devices");
     for (int j = 0; j < i; ++j)
@@ -242,7 +254,8 @@ int sgnx_batch_rhs(sgnx_batch* b, const double* const* U, double* const* dU, con
                          hipMemcpyHostToDevice, b->stream));
   BCHK(b, hipMemcpyAsync(b->d_ts, b->h_ts, ((size_t)n + 1) * sizeof(long long),
                          hipMemcpyHostToDevice, b->stream));
-  BCHK(b, launch_rhs_stage_batch(b->d_mem, b->d_ts, n, total, kPlain, b->waves3 & 1, b->stream));
+  BCHK(b, launch_rhs_stage_batch(b->d_mem, b->d_ts, n, total, kPlain, b->waves3 & 1, b->stream,
+                                 false, batch_hxr(b)));
   BCHK(b, hipStreamSynchronize(b->stream));
   return SGNX_OK;
 }
@@ -275,7 +288,7 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
     BCHK(b, hipMemcpyAsync(b->d_ts, b->h_ts, ((size_t)n + 1) * sizeof(long long),
                            hipMemcpyHostToDevice, b->stream));
     BCHK(b, launch_rhs_stage_batch(b->d_mem, b->d_ts, n, total, kPlain, b->waves3 & 1,
-                                   b->stream));
+                                   b->stream, false, batch_hxr(b)));
     // the copies above read the pinned tables asynchronously; the steps below
     // refill them
     BCHK(b, hipStreamSynchronize(b->stream));
@@ -357,6 +370,10 @@ int sgnx_batch_integrate(sgnx_batch* b, const double* t0, const double* tEnd, co
 
 int sgnx_autotune(sgnx_solver* s, int reps, int* rows_out, int* waves3_out) {
   if (!s || reps < 1) return SGNX_EINVAL;
+  if (s->ph.nManning != 0.0) {
+    s->err = "autotune: the row sweep has no Manning friction term";
+    return SGNX_EUNSUPPORTED;
+  }
   if (hipSetDevice(s->device) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess)
     return SGNX_EDEVICE;
   EventPair ev;
@@ -450,12 +467,12 @@ int sgnx_batch_autotune(sgnx_batch* b, int reps, int* rows_out, int* waves3_out)
   };
   auto timed = [&](int k, bool three, long long total, float* ms) -> hipError_t {
     hipError_t e = launch_rhs_stage_batch(b->d_mem + k * nn, b->d_ts, n, total, k, three,
-                                          b->stream);
+                                          b->stream, false, batch_hxr(b));
     if (e != hipSuccess) return e;
     if ((e = hipEventRecord(ev.a, b->stream)) != hipSuccess) return e;
     for (int r = 0; r < reps; ++r)
       if ((e = launch_rhs_stage_batch(b->d_mem + k * nn, b->d_ts, n, total, k, three,
-                                      b->stream)) != hipSuccess)
+                                      b->stream, false, batch_hxr(b))) != hipSuccess)
         return e;
     if ((e = hipEventRecord(ev.b, b->stream)) != hipSuccess) return e;
     if ((e = hipEventSynchronize(ev.b)) != hipSuccess) return e;
@@ -675,9 +692,10 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
     const auto* dVT = reinterpret_cast<const long long*>(dptr(oVT));
     BCHK(b, launch_gauge_bar_batch(reinterpret_cast<const GaugeMember*>(dptr(oG)), A, b->stream));
     if (Rn > 0) {
-      BCHK(b, launch_rhs_stage_batch(dR1, dRT, Rn, rAcc, kPlain, b->waves3 & 1, b->stream));
+      BCHK(b, launch_rhs_stage_batch(dR1, dRT, Rn, rAcc, kPlain, b->waves3 & 1, b->stream, false,
+                                     batch_hxr(b)));
       BCHK(b, launch_rhs_stage_batch(dR2, dRT, Rn, rAcc, kAxpy, (b->waves3 >> 1) & 1, b->stream,
-                                     true));
+                                     true, batch_hxr(b)));
       BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC3)), dCT, Rn, cAcc,
                                 b->stream));
     }

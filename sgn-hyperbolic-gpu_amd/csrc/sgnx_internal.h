@@ -56,6 +56,8 @@ struct Geometry {
   int variant;               // RHS kernel: 0 element-row tiles, 1 row sweep
   int rows;                  // row sweep: element rows per workgroup
   int waves3;                // row sweep: bit m set = stage mode m built for 3 waves/SIMD
+  int hxr;                   // Hx, Hy equal (D Hnod) / J bit for bit: the row sweep forms
+                             // them from Hnod instead of reading them
 };
 
 // Inflow ghost time factors for one RHS evaluation: sin(w t), cos(w t),
@@ -125,7 +127,7 @@ hipError_t launch_finish(const double* part, long long nb, double* out, hipStrea
 // their sum); `mode` as launch_rhs_stage, for all members.
 hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileStart, int nMem,
                                   long long totalTiles, int mode, bool waves3, hipStream_t st,
-                                  bool keep = false);
+                                  bool keep = false, bool hxr = false);
 // launch_finish for every member: member m's partials (mem[m].sa.part, its
 // nTiles) into out[2m], out[2m + 1]
 hipError_t launch_finish_batch(const BatchMember* mem, int nMem, double* out, hipStream_t st);
@@ -136,6 +138,10 @@ long long sweep_tiles(const Geometry& g);
 // Hx, Hy = (D . Hnod) / J per element line (sgn2d.cpp:54-71)
 hipError_t launch_bathy_grad(const Geometry& g, const double* Hnod, double* Hx, double* Hy,
                              hipStream_t st);
+// *bad |= 1 where Hx, Hy differ from (D . Hnod) / J in any bit (bad: device int,
+// cleared by the caller)
+hipError_t launch_bathy_check(const Geometry& g, const double* Hnod, const double* Hx,
+                              const double* Hy, int* bad, hipStream_t st);
 // out = u + a1*k1
 hipError_t launch_comb1(double* out, const double* u, double a1, const double* k1, size_t n,
                         hipStream_t st);

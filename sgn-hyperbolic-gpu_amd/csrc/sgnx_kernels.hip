@@ -353,16 +353,21 @@ struct NodeExtra {
   double x0[(MODE == kCand || MODE == kErr) ? NV : 1], x1[MODE == kErr ? NV : 1];
 };
 
-// Uniform per-variable base pointers: a node's value is base[v][node] with a
-// 32-bit node offset, so loads use a scalar base plus a 32-bit lane offset
-// instead of one 64-bit lane address per operand.
+// A five-variable operand by one base pointer: variable v of node n is
+// base[n + v Nd] with a 32-bit index (the grid has at most 2^28 nodes per
+// variable, so 5 Nd < 2^32). One base per operand instead of five plane
+// pointers keeps the launch's uniform values within the scalar registers.
 struct VarPtrs {
-  const double* p[NV];
+  const double* base;
+  unsigned nd;
+  __device__ __forceinline__ double at(int v, unsigned node) const {
+    return base[node + (unsigned)v * nd];
+  }
 };
 __device__ __forceinline__ VarPtrs var_ptrs(const double* base, long long Nd) {
   VarPtrs r;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) r.p[v] = base ? base + (long long)v * Nd : nullptr;
+  r.base = base;
+  r.nd = (unsigned)Nd;
   return r;
 }
 
@@ -374,8 +379,8 @@ template <int MODE>
 __device__ __forceinline__ void stage_load(const StagePtrs& P, unsigned node, StageRaw<MODE>& R) {
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    R.u[v] = P.u.p[v][node];
-    if (StageRaw<MODE>::kHasK) R.k[v] = P.k.p[v][node];
+    R.u[v] = P.u.at(v, node);
+    if (StageRaw<MODE>::kHasK) R.k[v] = P.k.at(v, node);
   }
 }
 
@@ -383,8 +388,8 @@ template <int MODE>
 __device__ __forceinline__ void extra_load(const StagePtrs& P, unsigned node, NodeExtra<MODE>& E) {
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    if (MODE == kCand || MODE == kErr) E.x0[v] = P.x0.p[v][node];
-    if (MODE == kErr) E.x1[v] = P.x1.p[v][node];
+    if (MODE == kCand || MODE == kErr) E.x0[v] = P.x0.at(v, node);
+    if (MODE == kErr) E.x1[v] = P.x1.at(v, node);
   }
 }
 
@@ -432,11 +437,16 @@ __device__ __forceinline__ unsigned node_of(int Nx, int ex, int ey, int a, int b
 
 // One workgroup's strip x chunk `tile` of one grid (the body of the plain and
 // the batched launches).
-template <int TW, int MODE, int WAVES, bool KEEP = false>
+template <int TW, int MODE, int WAVES, bool KEEP = false, bool HXR = false>
 __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double* __restrict__ dU,
-                                           const Geometry& g, const Phys& ph, const GhostTime& gt,
+                                           const Geometry& g, const Phys& phIn, const GhostTime& gt,
                                            const Stage& sa, int nStrips, int tile) {
   constexpr int NT = Sweep<TW>::NT, NXI = Sweep<TW>::NXI, NYI = Sweep<TW>::NYI;
+  // the row sweep runs without the Manning friction term (the launchers
+  // refuse it; solvers with friction use the element-row tiles), so its
+  // branch is compiled out
+  Phys ph = phIn;
+  ph.nManning = 0.0;
   __shared__ double sU[NV][NT];
   __shared__ double sF[NV][NT];
   __shared__ double sG[NV][NT];
@@ -445,6 +455,9 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
   __shared__ double xJ[2][NV][NXI], xhJ[2][NXI];
   __shared__ double yJ[2][2][NV][NYI], yhJ[2][2][NYI];
   __shared__ double sD[NPP], sgL[NP], sgR[NP];
+  // HXR: the still depth at the row's nodes, from which phase C forms Hx, Hy
+  // (sgn2d.cpp:54-71) instead of reading them
+  __shared__ double sH[HXR ? NT : 1];
 
   if (threadIdx.x < NPP) sD[threadIdx.x] = g.ops->D[threadIdx.x];
   if (threadIdx.x < NP) {
@@ -596,7 +609,8 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     const unsigned gn = node_of(Nx, ex0, ey0, 0, 0) + t;
     stage_load<MODE>(P, gn, pR);
     extra_load<MODE>(P, gn, pE);
-    pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
+    pH = g.Hnod[gn];
+    if (!HXR) { pHx = g.Hx[gn]; pHy = g.Hy[gn]; }
   }
   // scaled embedded error of this work-item's nodes (kErr)
   double eS = 0.0, eC = 0.0, eM = 0.0;
@@ -604,7 +618,8 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
   for (int r = ey0; r < ey1; ++r) {
     const unsigned rowBase = node_of(Nx, ex0, r, 0, 0);
     // ---- A ---------------------------------------------------------------------
-    double S[NV], qu = 0.0, qv = 0.0;
+    double S[NV], qu = 0.0, qv = 0.0, qw = 0.0;
+    SrcPart sp{0.0, 0.0, 0.0};
     // per-node operands phase C needs beyond the RHS (rk.cpp:106-108, :119-120,
     // sums evaluated left to right as there):
     //   kCand: c0 = u, c1 = 2/9 kc + 1/3 k, c2 = 7/24 kc + 0.25 k
@@ -629,7 +644,7 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
       if (KEEP && sa.yOut) {
         // the stage input, kept for a later reverse sweep
 #pragma unroll
-        for (int v = 0; v < NV; ++v) (sa.yOut + (long long)v * Nd)[rowBase + t] = Un[v];
+        for (int v = 0; v < NV; ++v) sa.yOut[rowBase + t + (unsigned)v * (unsigned)Nd] = Un[v];
       }
       const double Hn = pH, Hxn = pHx, Hyn = pHy;
 #pragma unroll
@@ -646,7 +661,14 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
       const Prim q = prims(Un);
       qu = q.u;
       qv = q.v;
-      source(Un, q, Hn, Hxn, Hyn, ph, S);
+      if (HXR) {
+        // the gradient terms wait for phase C
+        sp = source_part(Un, q, Hn, ph);
+        qw = q.w;
+        sH[t] = Hn;
+      } else {
+        source(Un, q, Hn, Hxn, Hyn, ph, S);
+      }
       double F[NV], G[NV];
       flux_x(Un, q, Hn, ph, F);
       flux_y(Un, q, Hn, ph, G);
@@ -667,7 +689,8 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
       const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
       stage_load<MODE>(P, gn, pR);
       extra_load<MODE>(P, gn, pE);
-      pH = g.Hnod[gn]; pHx = g.Hx[gn]; pHy = g.Hy[gn];
+      pH = g.Hnod[gn];
+      if (!HXR) { pHx = g.Hx[gn]; pHy = g.Hy[gn]; }
     }
     __syncthreads();
 
@@ -741,12 +764,18 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
       const int ixL = j * NP + b, ixR = (j + 1) * NP + b, iy = j * NP + a;
       const int bot = r & 1, top = (r + 1) & 1;
       const int rowx = j * NPP + NP * b, coly = j * NPP + a;
-      double dhx = 0.0, dhy = 0.0;
+      double dhx = 0.0, dhy = 0.0, dHx = 0.0, dHy = 0.0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         dhx = dhx + Da[k] * sU[IH][rowx + k];
         dhy = dhy + Db[k] * sU[IH][coly + NP * k];
+        if (HXR) {
+          // sgn2d.cpp:54-71: (D H) / J along the node's x and y lines
+          dHx = dHx + Da[k] * sH[rowx + k];
+          dHy = dHy + Db[k] * sH[coly + NP * k];
+        }
       }
+      if (HXR) source_rest(sp, qu, qv, qw, div_const(dHx, g.dx), div_const(dHy, g.dy), ph, S);
       dhx = dhx + (xhJ[1][ixL] * gLa + xhJ[0][ixR] * gRa);
       dhy = dhy + (yhJ[bot][1][iy] * gLb + yhJ[top][0][iy] * gRb);
       const double hx = div_const(dhx, g.dx), hy = div_const(dhy, g.dy);
@@ -767,15 +796,13 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
         double o = -div_const(dfx + cx, g.dx) - div_const(dgy + cy, g.dy);
         if (v == IHP) o = o - Bx - By;
         const double k4 = o + pick(S, v);
-        // per-variable pointers formed here (a loop-indexed pointer array
-        // would live in scratch memory)
-        const long long vo = (long long)v * Nd;
+        const unsigned vi = gn + (unsigned)v * (unsigned)Nd;
         if (MODE == kCand) {
           // rk.cpp:106-108 and the first three terms of :119-120 (k4 is k3 here)
-          (sa.uNew + vo)[gn] = pick(c0, v) + sa.dt * (pick(c1, v) + 4.0 / 9.0 * k4);
-          (sa.zOut + vo)[gn] = pick(c2, v) + 1.0 / 3.0 * k4;
+          sa.uNew[vi] = pick(c0, v) + sa.dt * (pick(c1, v) + 4.0 / 9.0 * k4);
+          sa.zOut[vi] = pick(c2, v) + 1.0 / 3.0 * k4;
         } else {
-          (dU + vo)[gn] = k4;
+          dU[vi] = k4;
         }
         if (MODE == kErr) {
           // rk.cpp:119-129 for this entry
@@ -819,19 +846,19 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
   }
 }
 
-template <int TW, int MODE, int WAVES, bool KEEP = false>
+template <int TW, int MODE, int WAVES, bool KEEP = false, bool HXR = false>
 __global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
 k_rhs_sweep(const double* __restrict__ U, double* __restrict__ dU, Geometry g, Phys ph,
             GhostTime gt, Stage sa, int nStrips) {
-  sweep_tile<TW, MODE, WAVES, KEEP>(U, dU, g, ph, gt, sa, nStrips,
-                                    xcd_tile(blockIdx.x, gridDim.x));
+  sweep_tile<TW, MODE, WAVES, KEEP, HXR>(U, dU, g, ph, gt, sa, nStrips,
+                                         xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // Many grids in one launch: the workgroups of every member's tiles, packed
 // back to back (tileStart[m] = first packed index of member m, ascending);
 // each member keeps its own tile decomposition, so its error partials, and
 // everything else, equal its own launch's.
-template <int TW, int MODE, int WAVES, bool KEEP = false>
+template <int TW, int MODE, int WAVES, bool KEEP = false, bool HXR = false>
 __global__ void __launch_bounds__(Sweep<TW>::NT, WAVES)
 k_rhs_sweep_batch(const BatchMember* __restrict__ mem, const long long* __restrict__ tileStart,
                   int nMem) {
@@ -845,7 +872,7 @@ k_rhs_sweep_batch(const BatchMember* __restrict__ mem, const long long* __restri
   const long long tile = w - tileStart[lo];
   // a grid larger than the members' tiles (never launched so) does nothing
   if (tile >= M.nTiles) return;
-  sweep_tile<TW, MODE, WAVES, KEEP>(M.U, M.dU, M.g, M.ph, M.gt, M.sa, M.nStrips, (int)tile);
+  sweep_tile<TW, MODE, WAVES, KEEP, HXR>(M.U, M.dU, M.g, M.ph, M.gt, M.sa, M.nStrips, (int)tile);
 }
 
 // Hx, Hy at every node (sgn2d.cpp:54-71).
@@ -866,6 +893,29 @@ k_bathy_grad(const double* __restrict__ H, double* __restrict__ Hx, double* __re
   }
   Hx[gn] = dHx / Jx;
   Hy[gn] = dHy / Jy;
+}
+
+// Whether Hx, Hy hold (D Hnod) / J (sgn2d.cpp:54-71, the arithmetic of
+// k_bathy_grad) bit for bit at every node: any difference sets *bad.
+__global__ void __launch_bounds__(kThreads)
+k_bathy_check(const double* __restrict__ H, const double* __restrict__ Hx,
+              const double* __restrict__ Hy, const OpTables* __restrict__ T, long long Nd,
+              double Jx, double Jy, int* bad) {
+  const long long gn = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (gn >= Nd) return;
+  const long long e = gn / NPP;
+  const int loc = (int)(gn - e * NPP);
+  const int b = loc / NP, a = loc - NP * b;
+  const double* He = H + e * NPP;
+  double dHx = 0.0, dHy = 0.0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    dHx = dHx + T->D[a * NP + k] * He[k + NP * b];
+    dHy = dHy + T->D[b * NP + k] * He[a + NP * k];
+  }
+  if (__double_as_longlong(dHx / Jx) != __double_as_longlong(Hx[gn]) ||
+      __double_as_longlong(dHy / Jy) != __double_as_longlong(Hy[gn]))
+    atomicOr(bad, 1);
 }
 
 // ---- RK stage arithmetic (src/solver/rk.cpp) ----------------------------------
@@ -1098,6 +1148,34 @@ long long sweep_tiles(const Geometry& g) {
   return strips * chunks;
 }
 
+namespace {
+
+// one row-sweep launch of a (mode, waves, keep) build, with the still-depth
+// gradients read (hxr false) or formed from Hnod (hxr true)
+template <int M, int W, bool K>
+void sweep_launch(bool hxr, dim3 grid, dim3 block, hipStream_t st, const double* U, double* dU,
+                  const Geometry& g, const Phys& ph, GhostTime gt, const Stage& sa, int strips) {
+  if (hxr)
+    hipLaunchKernelGGL((k_rhs_sweep<kTileW, M, W, K, true>), grid, block, 0, st, U, dU, g, ph, gt,
+                       sa, strips);
+  else
+    hipLaunchKernelGGL((k_rhs_sweep<kTileW, M, W, K, false>), grid, block, 0, st, U, dU, g, ph, gt,
+                       sa, strips);
+}
+
+template <int M, int W, bool K>
+void sweep_launch_batch(bool hxr, dim3 grid, dim3 block, hipStream_t st, const BatchMember* mem,
+                        const long long* tileStart, int nMem) {
+  if (hxr)
+    hipLaunchKernelGGL((k_rhs_sweep_batch<kTileW, M, W, K, true>), grid, block, 0, st, mem,
+                       tileStart, nMem);
+  else
+    hipLaunchKernelGGL((k_rhs_sweep_batch<kTileW, M, W, K, false>), grid, block, 0, st, mem,
+                       tileStart, nMem);
+}
+
+}  // namespace
+
 hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const Stage& sa,
                             double* dU, GhostTime gt, hipStream_t st) {
   const int strips = (g.Nx + kTileW - 1) / kTileW;
@@ -1107,10 +1185,11 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
   // waves per SIMD the register allocation targets, per stage mode (bit m of
   // g.waves3 set: 3 waves, else 2)
   const bool w3 = (g.waves3 >> mode) & 1;
-  if (mode < kPlain || mode > kErr) return hipErrorInvalidValue;
-#define SGNX_SWEEP(M, W, K)                                                             \
-  hipLaunchKernelGGL((k_rhs_sweep<kTileW, M, W, K>), grid, block, 0, st, sa.u, dU, g, ph, gt, sa, \
-                     strips)
+  if (mode < kPlain || mode > kErr || ph.nManning != 0.0) return hipErrorInvalidValue;
+  // forming Hx, Hy costs the plain build (three waves per SIMD) registers
+  // it does not have; the stage builds take it
+  const bool hxr = g.hxr != 0 && mode != kPlain;
+#define SGNX_SWEEP(M, W, K) sweep_launch<M, W, K>(hxr, grid, block, st, sa.u, dU, g, ph, gt, sa, strips)
   const bool keep = sa.yOut != nullptr;
   if (mode == kPlain) { if (w3) SGNX_SWEEP(kPlain, 3, false); else SGNX_SWEEP(kPlain, 2, false); }
   else if (mode == kAxpy) {
@@ -1128,22 +1207,18 @@ int sweep_strips(const Geometry& g) { return (g.Nx + kTileW - 1) / kTileW; }
 
 hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileStart, int nMem,
                                   long long totalTiles, int mode, bool waves3, hipStream_t st,
-                                  bool keep) {
+                                  bool keep, bool hxr) {
   if (nMem < 1 || totalTiles < 1 || totalTiles > 0x7fffffffLL) return hipErrorInvalidValue;
   if (mode < kPlain || mode > kErr) return hipErrorInvalidValue;
   const dim3 grid((unsigned)totalTiles), block(Sweep<kTileW>::NT);
-#define SGNX_SWEEPB(M, W) \
-  hipLaunchKernelGGL((k_rhs_sweep_batch<kTileW, M, W>), grid, block, 0, st, mem, tileStart, nMem)
-#define SGNX_SWEEPBK(M, W) \
-  hipLaunchKernelGGL((k_rhs_sweep_batch<kTileW, M, W, true>), grid, block, 0, st, mem, tileStart, \
-                     nMem)
-  if (mode == kPlain) { if (waves3) SGNX_SWEEPB(kPlain, 3); else SGNX_SWEEPB(kPlain, 2); }
-  else if (mode == kAxpy && keep) { if (waves3) SGNX_SWEEPBK(kAxpy, 3); else SGNX_SWEEPBK(kAxpy, 2); }
-  else if (mode == kAxpy) { if (waves3) SGNX_SWEEPB(kAxpy, 3); else SGNX_SWEEPB(kAxpy, 2); }
-  else if (mode == kCand && keep) { if (waves3) SGNX_SWEEPBK(kCand, 3); else SGNX_SWEEPBK(kCand, 2); }
-  else if (mode == kCand) { if (waves3) SGNX_SWEEPB(kCand, 3); else SGNX_SWEEPB(kCand, 2); }
-  else { if (waves3) SGNX_SWEEPB(kErr, 3); else SGNX_SWEEPB(kErr, 2); }
-#undef SGNX_SWEEPBK
+  hxr = hxr && mode != kPlain;
+#define SGNX_SWEEPB(M, W, K) sweep_launch_batch<M, W, K>(hxr, grid, block, st, mem, tileStart, nMem)
+  if (mode == kPlain) { if (waves3) SGNX_SWEEPB(kPlain, 3, false); else SGNX_SWEEPB(kPlain, 2, false); }
+  else if (mode == kAxpy && keep) { if (waves3) SGNX_SWEEPB(kAxpy, 3, true); else SGNX_SWEEPB(kAxpy, 2, true); }
+  else if (mode == kAxpy) { if (waves3) SGNX_SWEEPB(kAxpy, 3, false); else SGNX_SWEEPB(kAxpy, 2, false); }
+  else if (mode == kCand && keep) { if (waves3) SGNX_SWEEPB(kCand, 3, true); else SGNX_SWEEPB(kCand, 2, true); }
+  else if (mode == kCand) { if (waves3) SGNX_SWEEPB(kCand, 3, false); else SGNX_SWEEPB(kCand, 2, false); }
+  else { if (waves3) SGNX_SWEEPB(kErr, 3, false); else SGNX_SWEEPB(kErr, 2, false); }
 #undef SGNX_SWEEPB
   return hipGetLastError();
 }
@@ -1171,6 +1246,14 @@ hipError_t launch_rhs(const Geometry& g, const Phys& ph, const double* U, double
   if (tiles > 0x7fffffffLL) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_rhs<kTileW>, dim3((unsigned)tiles), dim3(Tile<kTileW>::NT), 0, st, U, dU,
                      g, ph, gt, tpr);
+  return hipGetLastError();
+}
+
+hipError_t launch_bathy_check(const Geometry& g, const double* Hnod, const double* Hx,
+                              const double* Hy, int* bad, hipStream_t st) {
+  const long long nb = (g.Nd + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL(k_bathy_check, dim3((unsigned)nb), dim3(kThreads), 0, st, Hnod, Hx, Hy,
+                     g.ops, g.Nd, g.Jx, g.Jy, bad);
   return hipGetLastError();
 }
 

@@ -91,23 +91,47 @@ __device__ __forceinline__ void interface_flux(const double* UL, const double* U
 }
 
 // sgn_flux2d.h:122-143 (source; the Manning branch uses the device cbrt, so
-// it is the one place that is not bitwise against the host libm)
-__device__ __forceinline__ void source(const double* U, const Prim& q, double Hl, double Hx,
-                                       double Hy, const Phys& ph, double* S) {
+// it is the one place that is not bitwise against the host libm). Split where
+// the still-depth gradients Hx, Hy enter: source_part needs only the state
+// and the nodal depth, source_rest adds the gradient terms, so a kernel can
+// form Hx, Hy after the state has been consumed. source() is the two in turn.
+struct SrcPart {
+  double bath;   // g (h - H) + gamma p
+  double pw;     // gamma p
+  double cf;     // Manning: n^2 g |u| / cbrt(h)
+};
+
+__device__ __forceinline__ SrcPart source_part(const double* U, const Prim& q, double Hl,
+                                               const Phys& ph) {
   const double h = U[IH];
-  const double bath = ph.g * (h - Hl) + ph.gamma * q.p;
-  S[IH] = 0.0;
-  S[IHU] = bath * Hx;
-  S[IHV] = bath * Hy;
-  S[IHW] = ph.gamma * q.p;
-  S[IHP] = ph.m2c2 * (q.w + q.u * Hx + q.v * Hy);
+  SrcPart sp;
+  sp.bath = ph.g * (h - Hl) + ph.gamma * q.p;
+  sp.pw = ph.gamma * q.p;
+  sp.cf = 0.0;
   if (ph.nManning != 0.0) {
     const double n2 = ph.nManning * ph.nManning;
     const double vmag = sqrt(q.u * q.u + q.v * q.v);
-    const double cf = n2 * ph.g * vmag / cbrt(h);
-    S[IHU] -= cf * q.u;
-    S[IHV] -= cf * q.v;
+    sp.cf = n2 * ph.g * vmag / cbrt(h);
   }
+  return sp;
+}
+
+__device__ __forceinline__ void source_rest(const SrcPart& sp, double qu, double qv, double qw,
+                                            double Hx, double Hy, const Phys& ph, double* S) {
+  S[IH] = 0.0;
+  S[IHU] = sp.bath * Hx;
+  S[IHV] = sp.bath * Hy;
+  S[IHW] = sp.pw;
+  S[IHP] = ph.m2c2 * (qw + qu * Hx + qv * Hy);
+  if (ph.nManning != 0.0) {
+    S[IHU] -= sp.cf * qu;
+    S[IHV] -= sp.cf * qv;
+  }
+}
+
+__device__ __forceinline__ void source(const double* U, const Prim& q, double Hl, double Hx,
+                                       double Hy, const Phys& ph, double* S) {
+  source_rest(source_part(U, q, Hl, ph), q.u, q.v, q.w, Hx, Hy, ph, S);
 }
 
 }  // namespace sgnx

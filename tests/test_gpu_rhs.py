@@ -108,6 +108,35 @@ def test_rhs_manning_within_tolerance():
     assert np.linalg.norm(Ug - Uo) <= 1e-12 * np.linalg.norm(Uo)
 
 
+def test_manning_solver_keeps_to_the_tile_kernel():
+    """The row sweep has no friction term: a solver with friction runs the
+    element-row tiles (the test above), and refuses the row sweep, a batch
+    and the autotuner with SGNX_EUNSUPPORTED, leaving itself usable."""
+    import dataclasses
+    import sgnx
+    from sgnx import _lib as L
+    from sgnx import cases
+    c = cases.gate(Nx=11, Ny=7, bc=("periodic", "periodic", "wall", "wall"))
+    c = dataclasses.replace(c, par=cases.Params(c.par.g, c.par.c, c.par.gamma, 0.03))
+    s = sgnx.SGN2D(**c.solver_kwargs())
+    with pytest.raises(L.SgnxError, match="Manning") as e:
+        s.set_rhs_kernel(1, 8)
+    assert e.value.code == L.SGNX_EUNSUPPORTED
+    with pytest.raises(L.SgnxError, match="Manning"):
+        sgnx.Batch([s])
+    s.upload(c.U0)
+    with pytest.raises(L.SgnxError, match="Manning"):
+        s.autotune()
+    o = OL.Oracle.from_case(c)
+    got, want = s.rhs(c.U0), o.rhs(c.U0)
+    o.close()
+    s.close()
+    nd = c.nd
+    for v in range(5):
+        scale = max(np.abs(want[v * nd:(v + 1) * nd]).max(), 1e-300)
+        assert np.abs(got[v * nd:(v + 1) * nd] - want[v * nd:(v + 1) * nd]).max() <= 1e-13 * scale
+
+
 def test_rhs_flat_no_depth_bitwise():
     """Flat bottom given as no still depth at all (the reference's H == 0)."""
     from sgnx import cases
