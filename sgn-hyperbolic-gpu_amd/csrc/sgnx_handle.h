@@ -32,6 +32,12 @@ struct sgnx_solver {
   double *d_tmp = nullptr, *d_uNew = nullptr;
   double *d_part = nullptr, *d_out = nullptr;
   double* h_out = nullptr;   // pinned, 2 doubles
+  // the adaptive loop's per-attempt result, written by the device straight
+  // into host memory: (sum, max, sequence number); h_post is the host view,
+  // d_post the device address of the same memory
+  double* h_post = nullptr;
+  double* d_post = nullptr;
+  unsigned long long postSeq = 0;
   double inflowOmega = 0.0;
   bool hasInflow = false;
   std::string err;

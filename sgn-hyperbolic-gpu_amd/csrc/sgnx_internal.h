@@ -122,6 +122,10 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
                             double* dU, GhostTime gt, hipStream_t st);
 // out[0] = sum, out[1] = max of nb (sum, carry, max) partial triples, fixed order
 hipError_t launch_finish(const double* part, long long nb, double* out, hipStream_t st);
+// the same into host memory mapped for the device (device address `host`):
+// host[0] = sum, host[1] = max, then host[2] = seq (as 64-bit integer bits)
+hipError_t launch_finish_post(const double* part, long long nb, double* host,
+                              unsigned long long seq, hipStream_t st);
 // The row sweep over every member of a batch in one launch (device arrays:
 // mem[nMem]; tileStart[m] = sum of the tiles of members before m, totalTiles
 // their sum); `mode` as launch_rhs_stage, for all members.

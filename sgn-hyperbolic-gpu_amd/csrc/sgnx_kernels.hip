@@ -998,6 +998,22 @@ k_finish(const double* __restrict__ part, int nb, double* __restrict__ out) {
   finish_block(part, nb, out);
 }
 
+// k_finish writing straight into host memory mapped for the device: the
+// (sum, max) pair, then, after a system-scope fence, the attempt's sequence
+// number, for which the host polls.
+__global__ void __launch_bounds__(kThreads)
+k_finish_post(const double* __restrict__ part, int nb, double* __restrict__ host,
+              unsigned long long seq) {
+  __shared__ double pair[2];
+  finish_block(part, nb, pair);
+  if (threadIdx.x == 0) {
+    host[0] = pair[0];
+    host[1] = pair[1];
+    __threadfence_system();
+    reinterpret_cast<volatile unsigned long long*>(host)[2] = seq;
+  }
+}
+
 // one block per member: member m's partials into out[2m], out[2m+1]
 __global__ void __launch_bounds__(kThreads)
 k_finish_batch(const BatchMember* __restrict__ mem, int nMem, double* __restrict__ out) {
@@ -1226,6 +1242,12 @@ hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileS
 hipError_t launch_finish_batch(const BatchMember* mem, int nMem, double* out, hipStream_t st) {
   if (nMem < 1) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_finish_batch, dim3((unsigned)nMem), dim3(kThreads), 0, st, mem, nMem, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_post(const double* part, long long nb, double* host,
+                              unsigned long long seq, hipStream_t st) {
+  hipLaunchKernelGGL(k_finish_post, dim3(1), dim3(kThreads), 0, st, part, (int)nb, host, seq);
   return hipGetLastError();
 }
 
