@@ -222,6 +222,41 @@ def shoal_focus(Nx=384, Ny=192, P=4, d_shoal=0.65, r_shoal=3.0):
                       "r_shoal": r_shoal})
 
 
+def tsunami_basin(Nx=1000, Ny=500, P=4, w_slope=5.0e4):
+    """The basin-scale tsunami of apps/tsunami_margin_mpi.cu:130-186: a
+    4000 x 2000 km periodic basin, 4000 m deep between two continental slopes
+    (tanh of half-width w_slope centred at x = 1300 km and at a curved eastern
+    margin 2700 km + 200 km cos(2 pi (y - ySrc) / Ly)) rising to a 150 m shelf;
+    a 2 m elliptical Gaussian uplift (semi-axes 50 and 200 km) at (2000, 1000)
+    km with the fluid at rest; c = 3 sqrt(g 4000). meta: the shelf window's
+    margin curve, the five gauges on the eastern shelf (:187-192) and the
+    adaptive tolerances."""
+    Lx, Ly = 4.0e6, 2.0e6
+    h_deep, h_shelf = 4000.0, 150.0
+    xw, xe, bend = 1.3e6, 2.7e6, 2.0e5
+    xs, ys, a_src, sx, sy = 2.0e6, 1.0e6, 2.0, 5.0e4, 2.0e5
+    par = Params(g=G, c=ALPHA * math.sqrt(G * h_deep), gamma=GAMMA)
+
+    def margin_x(y):
+        return xe + bend * np.cos(2.0 * math.pi * (y - ys) / Ly)
+
+    def depth_fn(x, y):
+        s_ = 0.5 * (np.tanh((x - xw) / w_slope) - np.tanh((x - margin_x(y)) / w_slope))
+        return h_shelf + (h_deep - h_shelf) * s_
+
+    bc = ("periodic",) * 4
+    depth = depth_arrays(P, Nx, Ny, Lx, Ly, depth_fn, bc)
+    x, y = node_coords(P, Nx, Ny, Lx, Ly)
+    dx, dy = (x - xs) / sx, (y - ys) / sy
+    h = depth["Hnod"] + a_src * np.exp(-(dx * dx + dy * dy))
+    z = np.zeros_like(h)
+    gy = np.array([ys + (k - 2) * 0.18 * Ly for k in range(5)])
+    gx = margin_x(gy) + 1.2e5
+    return Case("tsunami_basin", P, Nx, Ny, Lx, Ly, par, depth, bc, prim_to_state(h, z, z, z, z),
+                meta={"atol": 1e-8, "rtol": 1e-6, "cfl": 0.5, "margin_x": margin_x,
+                      "gauges_xy": (gx, gy), "source": a_src})
+
+
 # manufactured solution of apps/convergence2d.cpp:38-84 (phi = kx x + ky y - w t)
 MMS = {"g": 9.81, "gamma": 1.5, "alpha": 3.0, "L": 1.0, "k": 2.0 * math.pi, "omega": 1.0,
        "ah": 0.10, "au": 0.15, "av": 0.12, "aw": 0.05, "ap": 0.03}
