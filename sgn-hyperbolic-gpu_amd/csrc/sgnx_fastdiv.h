@@ -12,8 +12,11 @@
 // which is the IEEE quotient a/b bit for bit. The last step is evaluated as
 // -fma(-r, rh, -q1) so that a = -0 yields -0 like the division does.
 // Preconditions (checked): b > 0 with 2^-100 <= b <= 2^100 (else the caller
-// keeps plain division), and |a| in [2^-800, 2^800] or a == 0, so that no
-// intermediate underflows or overflows; any other a takes a / b.
+// keeps plain division), and |a| in [2^-900, 2^800] or a == 0. Then |a/b| >=
+// 2^-1000 is normal, the remainder's last bit (about 2^(e_a - 105)) is above
+// 2^-1074, and the rounding error of a*rl (at most 2^-1075 when it is
+// subnormal) is far below half an ulp of q1, so every step above holds. Any
+// other a (tiny, huge, infinite, NaN) takes a / b.
 //
 // Three fmas and one multiply (full-rate) against the ~10-instruction
 // scaled-reciprocal division sequence with its quarter-rate v_rcp_f64.
@@ -53,7 +56,7 @@ inline ConstDiv make_const_div(double b) {
 // a is in the range the fast sequence is exact for
 SGNX_HD inline bool div_fast_ok(double a) {
   const double aa = fabs(a);
-  return aa <= 0x1p+800 && (aa >= 0x1p-800 || aa == 0.0);
+  return aa <= 0x1p+800 && (aa >= 0x1p-900 || aa == 0.0);
 }
 
 SGNX_HD inline double div_fast(double a, const ConstDiv& d) {
@@ -66,18 +69,6 @@ SGNX_HD inline double div_fast(double a, const ConstDiv& d) {
 SGNX_HD inline double div_const(double a, const ConstDiv& d) {
   if (!d.fast || !div_fast_ok(a)) return a / d.b;
   return div_fast(a, d);
-}
-
-// Two quotients behind one range test (one branch instead of two on a GPU).
-SGNX_HD inline void div_const2(double a0, double a1, const ConstDiv& d0, const ConstDiv& d1,
-                               double& q0, double& q1) {
-  if ((d0.fast & d1.fast) && div_fast_ok(a0) && div_fast_ok(a1)) {
-    q0 = div_fast(a0, d0);
-    q1 = div_fast(a1, d1);
-  } else {
-    q0 = a0 / d0.b;
-    q1 = a1 / d1.b;
-  }
 }
 
 }  // namespace sgnx

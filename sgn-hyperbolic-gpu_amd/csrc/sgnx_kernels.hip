@@ -588,7 +588,15 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     unsigned n;
     return face_node(r, n);
   };
-  // the interface lanes' global operands, loaded one row ahead
+  // The interface lanes' global operands of row r + 1: the halo nodes and
+  // the bottom face of row r + 2. The stage launches issue them after their
+  // phase A of row r. The plain launch issues them at the start of phase C,
+  // which leaves phase C and the next phase A to cover their latency, brings
+  // the full loads of row r + 2 (issued after that phase A) close enough
+  // behind that fewer bottom-face lines have left L2, and shortens their
+  // register lifetime (measured 4 % faster; the stage launches were not
+  // faster this way).
+  constexpr bool kLateFace = MODE == kPlain;
   StageRaw<MODE> fR;
   double fH = 0.0;
   auto face_prefetch = [&](int r) {
@@ -626,11 +634,11 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     //   kErr:  c0 = u0, c2 = z
     double c0[NV], c1[NV], c2[NV];
     // the interface lane's global node state and face depth for this row
-    // (loaded together with, and just before, this row's node operands)
+    // (the plain launch takes them at the start of phase B instead)
     double SG[NV];
     double Hf = 0.0;
     const bool haveG = (isX || isY) && face_node_now(r);
-    if (isX || isY) {
+    if (!kLateFace && (isX || isY)) {
       Hf = fH;
       if (haveG) {
         stage_combine<MODE>(fR, sa.a, SG);
@@ -684,7 +692,7 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     // them above those uses: the loads then land in the registers the loop
     // carries, so no copy (and no wait for them) is needed at the back-edge.
     __builtin_amdgcn_sched_barrier(0);
-    if (r + 1 < ey1) face_prefetch(r + 1);
+    if (!kLateFace && r + 1 < ey1) face_prefetch(r + 1);
     if (isNode && r + 1 < ey1) {
       const unsigned gn = rowBase + (unsigned)Nx * NPP + t;
       stage_load<MODE>(P, gn, pR);
@@ -697,7 +705,14 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     // ---- B ---------------------------------------------------------------------
     if (isX || isY) {
       double W[NV];
-      if (haveG) {
+      if (kLateFace) {
+        Hf = fH;
+        if (haveG) {
+          stage_combine<MODE>(fR, sa.a, W);
+#pragma unroll
+          for (int v = 0; v < NV; ++v) W[v] = W[v] + 0.0;
+        }
+      } else if (haveG) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) W[v] = SG[v];
       }
@@ -756,6 +771,10 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     __syncthreads();
 
     // ---- C: dU of row r ---------------------------------------------------------
+    if (kLateFace) {
+      if (r + 1 < ey1) face_prefetch(r + 1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     if (isNode) {
       double Da[NP], Db[NP];
 #pragma unroll

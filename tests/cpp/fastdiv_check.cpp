@@ -42,9 +42,25 @@ int main(int argc, char** argv) {
       check(std::nextafter(a, -INFINITY));
       check((m + 0.5) * b);
     }
+    // tiny dividends (the scaled path and its subnormal-quotient cut-off):
+    // random significands with exponents from the subnormal range to 2^-880
+    for (long i = 0; i < n; ++i) {
+      const uint64_t mant = rng() & 0x000fffffffffffffULL, sign = rng() & 0x8000000000000000ULL;
+      const uint64_t ex = rng() % 145;          // biased exponents 0 (subnormal) .. 144 (2^-879)
+      check(from_bits(sign | (ex << 52) | mant));
+    }
+    // around the quotient's normal / subnormal boundary: a near 2^-1022 b
+    for (long i = 0; i < n / 4; ++i) {
+      double a = std::ldexp(b, -1022) * (1.0 + std::ldexp(U(rng), -(int)(rng() % 60)));
+      check(a);
+      check(std::nextafter(a, INFINITY));
+      check(-std::nextafter(a, 0.0));
+    }
     for (double a : {0.0, -0.0, (double)INFINITY, -(double)INFINITY, (double)NAN, 0x1p-1074, -0x1p-1074, 0x1p-1022,
-                     0x1p-800, 0x1.fffffffffffffp-801, 0x1p+800, 0x1.0000000000001p+800,
-                     0x1.fffffffffffffp+1023, 1.0, -1.0, b, -b})
+                     0x1p-800, 0x1.fffffffffffffp-801, 0x1p-900, 0x1.fffffffffffffp-901,
+                     -0x1.0000000000001p-900, 0x1p+800, 0x1.0000000000001p+800,
+                     0x1.fffffffffffffp+1023, 1.0, -1.0, b, -b, std::ldexp(b, -1022),
+                     std::ldexp(b, -1021), std::ldexp(b, -1023)})
       check(a);
   }
   std::printf("fastdiv: %ld tested, %ld mismatches\n", tested, bad);
