@@ -39,14 +39,15 @@ Also reported:
                 per-step launch and synchronisation costs dominate a solo run
   adjoint       a gradient at 1024x512 (taped run + reverse sweep) against the
                 plain run, with and without kept stage inputs
-  inversion     the gauge-inversion gradient (256x128, 2000 steps, 16 snapshots +
-                20 recomputation slots) checked by differences in 32 directions
+  inversion     the gauge-inversion gradient (256x128, 2000 steps, at most 40 states
+                held: planned binomial snapshots and recomputation slots) checked
+                by differences in 32 directions
   batched_adjoint  16 uneven members' gradients, one by one against one batch,
                 each checked by differences
   margin        the 2000x1000 margin workload (50 M nodes), 1000 adaptive steps
   margin_adjoint  a source-inversion gradient on the margin workload (100 steps,
-                10 snapshots + 10 recomputation slots of 2 GB each) checked by
-                differences in 3 directions
+                at most 20 states of 2 GB held, planned binomial schedule) checked
+                by differences in 3 directions
 
 usage: python bench.py [--gpus 1] [--steps 500] [--warmup 20] [--seed 0]
        python bench.py --profile-rhs N    (N RHS launches + a copy of known size,
@@ -374,7 +375,8 @@ def inversion_leg(args):
     0.5 |G - G_obs|^2 with respect to the initial surface eta0 and the still
     depth theta over --inversion-steps adaptive steps (atol 1e-10, rtol 1e-8),
     under a budget of 40
-    stored states (20 snapshots + 20 recomputation slots). G_obs is the same
+    states held at once (snapshots and recomputation slots, the snapshots at the
+    first sweep of the binomial schedule for the known step count). G_obs is the same
     run from 1.1 eta0. Reports its cost against the plain run, and the largest
     relative difference to central differences of the plain solver's replay
     of the taped steps over --inversion-dirs seeded smooth directions (held
@@ -402,12 +404,11 @@ def inversion_leg(args):
     t_fwd = time.perf_counter() - t0
     s.upload(c.U0)
     s.setGauges(nodes, steps)
-    s.tapeBegin(20)
+    s.tapeBeginPlan(steps, 40)
     t0 = time.perf_counter()
     st1 = s.integrateAdaptive(0.0, 1e9, dt0, opts)
     t_tape = time.perf_counter() - t0
     s.tapeEnd()
-    s.tapeSlots(20)
     G = s.gaugeLog()[1]
     t0 = time.perf_counter()
     U0b, thb = s.adjoint(None, G - G_obs)
@@ -441,8 +442,9 @@ def inversion_leg(args):
                         f"{nodes.size} gauges, {steps} accepted steps"),
            "accepted": st.accepted, "rejected": st.rejected,
            "same_step_counts": st.as_dict() == st1.as_dict(),
-           "stored_states": {"snapshots": held, "stride": stride, "slots_peak": peak,
-                             "budget": 40},
+           "stored_states": {"snapshots": held, "slots_peak": peak, "budget": 40,
+                             "schedule": "binomial, planned for the step count (snapshots "
+                                         "at its first sweep, the taped run)"},
            "recomputed_steps": recomputed,
            "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "adjoint_ms": 1e3 * t_adj,
            "adjoint_over_forward": t_adj / t_fwd,
@@ -456,7 +458,7 @@ def inversion_leg(args):
                             "replay of the taped steps, along seeded smooth directions (4 "
                             "Gaussians each for eta0 and theta, centred near gauges)")}
     log(f"[bench] inversion: {st.accepted} steps, forward {1e3 * t_fwd:.0f} ms, taped "
-        f"{1e3 * t_tape:.0f} ms, reverse {1e3 * t_adj:.0f} ms ({held} snapshots every {stride}, "
+        f"{1e3 * t_tape:.0f} ms, reverse {1e3 * t_adj:.0f} ms ({held} planned snapshots, "
         f"{peak} slots, {recomputed} steps recomputed) -> gradient/forward "
         f"{(t_tape + t_adj) / t_fwd:.2f}; FD over {len(errs)} directions: max rel err "
         f"{max(errs) if errs else float('nan'):.2e} ({t_fd:.1f}s)")
@@ -637,8 +639,7 @@ def margin_adjoint_leg(args, case):
     the gauge misfit 0.5 |G(eta0) - G_obs|^2 with respect to the initial
     surface over --margin-adjoint-steps adaptive steps, 16 gauges inside the
     uplift, G_obs the same run from 1.1 times the uplift. At 2 GB per state
-    the tape holds 10 snapshots and the sweep 10 more states (binomial
-    recomputation). Reports its cost against the plain run and the largest
+    the sweep holds at most 20 states (planned binomial schedule). Reports its cost against the plain run and the largest
     relative difference to fourth-order central differences of the plain
     solver's replay of the taped steps along --margin-adjoint-dirs seeded
     smooth directions (Gaussians 2-6 km wide near the gauges)."""
@@ -665,12 +666,11 @@ def margin_adjoint_leg(args, case):
     t_fwd = time.perf_counter() - t0
     s.upload(case.U0)
     s.setGauges(nodes, steps)
-    s.tapeBegin(10)
+    s.tapeBeginPlan(steps, 20)
     t0 = time.perf_counter()
     st1 = s.integrateAdaptive(0.0, 1e9, dt0, opts)
     t_tape = time.perf_counter() - t0
     s.tapeEnd()
-    s.tapeSlots(10)
     G = s.gaugeLog()[1]
     t0 = time.perf_counter()
     g_eta = s.adjoint(None, G - G_obs, theta=False)[:nd]
@@ -696,7 +696,7 @@ def margin_adjoint_leg(args, case):
     t_fd = time.perf_counter() - t0
     s.close()
     log(f"[bench] margin adjoint: {st.accepted} steps, forward {1e3 * t_fwd:.0f} ms, taped "
-        f"{1e3 * t_tape:.0f} ms, reverse {1e3 * t_adj:.0f} ms ({held} snapshots every {stride}, "
+        f"{1e3 * t_tape:.0f} ms, reverse {1e3 * t_adj:.0f} ms ({held} planned snapshots, "
         f"{peak} slots, {recomputed} steps recomputed) -> gradient/forward "
         f"{(t_tape + t_adj) / t_fwd:.2f}; FD over {len(errs)} directions: max rel err "
         f"{max(errs) if errs else float('nan'):.2e} ({t_fd:.1f}s)")
@@ -705,8 +705,9 @@ def margin_adjoint_leg(args, case):
                          f"{steps} accepted steps, atol 1e-8 rtol 1e-6"),
             "accepted": st.accepted, "rejected": st.rejected,
             "same_step_counts": st.as_dict() == st1.as_dict(),
-            "stored_states": {"snapshots": held, "stride": stride, "slots_peak": peak,
-                              "budget": 20, "bytes_per_state": 5 * 8 * nd},
+            "stored_states": {"snapshots": held, "slots_peak": peak, "budget": 20,
+                              "bytes_per_state": 5 * 8 * nd,
+                              "schedule": "binomial, planned for the step count"},
             "recomputed_steps": recomputed,
             "forward_ms": 1e3 * t_fwd, "taped_forward_ms": 1e3 * t_tape, "adjoint_ms": 1e3 * t_adj,
             "adjoint_over_forward": t_adj / t_fwd,

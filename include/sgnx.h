@@ -261,10 +261,21 @@ int sgnx_rhs_vjp(sgnx_solver* s, const double* U, const double* W, double* Ubar,
 /* Tape of adaptive runs: between tape_begin and tape_end, sgnx_integrate
  * (and sgnx_batch_integrate, for each member) records every accepted step (t, dt, gauge log row) and snapshots of step
  * input states, at most max_snapshots held (every K-th step; K doubles when
- * the budget is full). tape_info: steps recorded, stride K, snapshots held.
+ * the budget is full). tape_info: steps recorded, stride K (0: planned
+ * snapshots, sgnx_tape_begin_plan), snapshots held.
  * tape_replay: the recorded steps (fixed t, dt) on the resident state, with
  * gauge recording -- the map the adjoint differentiates. */
 int sgnx_tape_begin(sgnx_solver* s, long max_snapshots);
+/* Starts a tape like sgnx_tape_begin for a run of `steps` accepted steps
+ * (known in advance, e.g. a max_accepted bound) whose reverse sweep may hold
+ * `states` states at once, snapshots included: the snapshots are taken at the
+ * steps of the first sweep of the binomial schedule (Griewank's revolve) for
+ * that budget, and the sweep reverses segment j with states - j - 1 slots,
+ * which recomputes the fewest steps the budget allows (sgnx_tape_set_slots is
+ * not used). A run of fewer steps is reversed correctly with fewer
+ * snapshots; the steps of a longer run past the plan form one segment. No
+ * reference counterpart (the reference has no adjoint). */
+int sgnx_tape_begin_plan(sgnx_solver* s, long steps, long states);
 /* States the reverse sweep may hold besides the snapshots while it recomputes
  * a segment (-1, the default: all K - 1 of it). Fewer slots select a binomial
  * schedule (Griewank's revolve bound) inside each segment, plus two states of

@@ -210,6 +210,9 @@ class SGN2DHip {
   }
   // record the following integrateAdaptive calls (at most maxSnapshots states)
   void tapeBegin(long maxSnapshots) { detail::check(sgnx_tape_begin(s_, maxSnapshots), s_); }
+  void tapeBeginPlan(long steps, long states) {
+    detail::check(sgnx_tape_begin_plan(s_, steps, states), s_);
+  }
   void tapeEnd() { detail::check(sgnx_tape_end(s_), s_); }
   void tapeSlots(long slots) { detail::check(sgnx_tape_set_slots(s_, slots), s_); }
   // keep each taped step's stage inputs (two more states per step) so the

@@ -214,6 +214,29 @@ long best_split(long L, long c) {
   return best;
 }
 
+// The snapshot steps of a planned tape: the first forward sweep of the
+// binomial schedule for `steps` steps with `states` states held at once (the
+// start state included): p_0 = 0, p_{j+1} = p_j + best_split(steps - p_j, c - j)
+// with c = states - 1, until no slot is left or the remaining range fits its
+// free slots, whose every step is then a snapshot too (the taped run keeps
+// them for free). Reversing segment j with c - j free slots, last segment
+// first, then recomputes at most F(steps, c) - (steps - 1) steps.
+std::vector<long> plan_positions(long steps, long states) {
+  std::vector<long> at{0};
+  const long c = states - 1;
+  for (;;) {
+    const long j = (long)at.size() - 1, rem = steps - at.back();
+    if (rem < 2) break;
+    if (c - j >= rem - 1) {
+      for (long k = 1; k < rem; ++k) at.push_back(at.back() + 1);
+      break;
+    }
+    if (c - j < 1) break;
+    at.push_back(at.back() + best_split(rem, c - j));
+  }
+  return at;
+}
+
 int take_slot(sgnx_solver* s, AdjTape* tp, double** out) {
   if (tp->freeSeg.empty()) {
     double* p = nullptr;
@@ -305,7 +328,11 @@ AdjTape* tape_of(sgnx_solver* s) {
 
 void release_snapshots(AdjTape* tp) {
   for (double* p : tp->snap) tp->spare.push_back(p);
+  if (tp->lastIn && !tp->lastInSnap) tp->spare.push_back(tp->lastIn);
+  tp->lastIn = nullptr;
+  tp->lastInSnap = false;
   tp->snap.clear();
+  tp->snapAt.clear();
 }
 
 void release_stages(AdjTape* tp) {
@@ -333,9 +360,9 @@ int adjoint_begin(sgnx_solver* s, bool storeAll, AdjTape** out) {
   int rc;
   if ((rc = check_adjoint_support(s))) return rc;
   const long N = (long)tp->t.size();
-  if ((long)tp->snap.size() < (N + tp->K - 1) / tp->K)
+  if (N > 0 && (tp->snap.empty() || tp->snapAt.size() != tp->snap.size() || tp->snapAt[0] != 0))
     return fail(s, SGNX_EINVAL, "adjoint: the tape lacks snapshots");
-  if (storeAll && N > 0 && tp->K != 1)
+  if (storeAll && N > 0 && (long)tp->snap.size() != N)
     return fail(s, SGNX_EINVAL,
                 "adjoint: a batched sweep needs every step's snapshot (max_snapshots >= steps)");
   if ((rc = ensure_vjp_buffers(s, tp))) return rc;
@@ -408,6 +435,11 @@ int tape_step_begin(sgnx_solver* s) {
     tp->y2s.push_back(a);
     tp->y3s.push_back(b);
   }
+  if (tp->planStates > 0) {
+    // planned mode: the snapshot steps are fixed in advance
+    tp->pendingSnap = tp->nextPlan < tp->planAt.size() && tp->planAt[tp->nextPlan] == n;
+    return SGNX_OK;
+  }
   if (n % tp->K != 0 || (long)tp->snap.size() > n / tp->K) return SGNX_OK;
   while ((long)tp->snap.size() >= tp->maxSnap) {
     // the stage states are kept only while every step's snapshot is
@@ -415,11 +447,17 @@ int tape_step_begin(sgnx_solver* s) {
     release_stages(tp);
     // budget full: keep every other snapshot, double the stride
     std::vector<double*> keep;
+    std::vector<long> keepAt;
     for (size_t j = 0; j < tp->snap.size(); ++j) {
-      if (j % 2 == 0) keep.push_back(tp->snap[j]);
-      else tp->spare.push_back(tp->snap[j]);
+      if (j % 2 == 0) {
+        keep.push_back(tp->snap[j]);
+        keepAt.push_back(tp->snapAt[j]);
+      } else {
+        tp->spare.push_back(tp->snap[j]);
+      }
     }
     tp->snap.swap(keep);
+    tp->snapAt.swap(keepAt);
     tp->K *= 2;
     if (n % tp->K != 0) return SGNX_OK;
   }
@@ -432,13 +470,24 @@ int tape_accept(sgnx_solver* s, double t, double dt, long row) {
   tp->t.push_back(t);
   tp->dt.push_back(dt);
   tp->row.push_back(row);
-  if (tp->pendingSnap) {
+  if (tp->pendingSnap || tp->planStates > 0) {
     // no copy: the buffer holding the accepted step's input becomes the
-    // snapshot, and the solver takes a spare one for its next candidate
+    // snapshot, and the solver takes a spare one for its next candidate. A
+    // planned tape also holds every accepted step's input until the next
+    // acceptance (the last one is where the sweep starts).
     double* fresh = nullptr;
     const int rc = take_spare(s, tp, &fresh);
     if (rc) return rc;
-    tp->snap.push_back(s->d_uNew);
+    if (tp->planStates > 0) {
+      if (tp->lastIn && !tp->lastInSnap) tp->spare.push_back(tp->lastIn);
+      tp->lastIn = s->d_uNew;
+      tp->lastInSnap = tp->pendingSnap;
+    }
+    if (tp->pendingSnap) {
+      tp->snap.push_back(s->d_uNew);
+      tp->snapAt.push_back((long)tp->t.size() - 1);
+      if (tp->planStates > 0) ++tp->nextPlan;
+    }
     s->d_uNew = fresh;
     tp->pendingSnap = false;
   }
@@ -536,10 +585,28 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots) {
   tp->recording = true;
   tp->maxSnap = max_snapshots;
   tp->K = 1;
+  tp->planStates = 0;
+  tp->planAt.clear();
+  tp->nextPlan = 0;
   tp->t.clear();
   tp->dt.clear();
   tp->row.clear();
   tp->gnodes = s->gnodes;
+  return SGNX_OK;
+}
+
+int sgnx_tape_begin_plan(sgnx_solver* s, long steps, long states) {
+  if (!s) return SGNX_EINVAL;
+  if (steps < 1 || states < 1)
+    return fail(s, SGNX_EINVAL, "tape: the plan needs steps >= 1 and states >= 1");
+  std::vector<long> at = plan_positions(steps, states);
+  int rc;
+  if ((rc = sgnx_tape_begin(s, (long)at.size()))) return rc;
+  AdjTape* tp = s->tape;
+  tp->planStates = states;
+  tp->planAt.swap(at);
+  // stage inputs are kept only if every step has its snapshot
+  tp->stagesOff = (long)tp->planAt.size() < steps;
   return SGNX_OK;
 }
 
@@ -587,7 +654,7 @@ int sgnx_tape_info(const sgnx_solver* s, long* steps, long* stride, long* snapsh
   if (!s || !steps || !stride || !snapshots) return SGNX_EINVAL;
   const AdjTape* tp = s->tape;
   *steps = tp ? (long)tp->t.size() : 0;
-  *stride = tp ? tp->K : 1;
+  *stride = tp ? (tp->planStates > 0 ? 0 : tp->K) : 1;
   *snapshots = tp ? (long)tp->snap.size() : 0;
   return SGNX_OK;
 }
@@ -632,19 +699,34 @@ int sgnx_adjoint(sgnx_solver* s, const double* Ubar_final, const double* gauge_b
   const long N = (long)tp->t.size();
   const size_t n = s->n();
   // snapshot segments, last first; inside a segment the states are recomputed
-  // with at most `slots` held at once (binomial schedule when that is fewer
-  // than the segment needs)
-  const long K = tp->K;
-  const long c = tp->slots < 0 ? K : tp->slots;
-  if (c < K - 1)
+  // with at most c_j held at once (binomial schedule when that is fewer than
+  // the segment needs): the slots of sgnx_tape_set_slots, or in planned mode
+  // the planned states less the snapshots still held (segment j: j + 1)
+  const long S = (long)tp->snap.size();
+  auto seg_end = [&](long j) { return j + 1 < S ? tp->snapAt[j + 1] : N; };
+  auto free_slots = [&](long j) {
+    if (tp->planStates > 0) return std::max(0L, tp->planStates - (j + 1));
+    return tp->slots < 0 ? seg_end(j) - tp->snapAt[j] : tp->slots;
+  };
+  bool pairNeeded = false;
+  for (long j = 0; j < S; ++j)
+    if (free_slots(j) < seg_end(j) - tp->snapAt[j] - 1) pairNeeded = true;
+  if (pairNeeded)
     for (double** p : {&tp->pp[0], &tp->pp[1]})
       if ((rc = alloc_state(s, p, n))) return rc;
   tp->recomputed = 0;
   tp->peakSlots = 0;
   tp->inUse = 0;
-  for (long j = (N + K - 1) / K - 1; j >= 0; --j) {
-    const long n0 = j * K, n1 = std::min(n0 + K, N);
-    if ((rc = reverse_range(s, tp, tp->snap[j], n0, n1 - n0, c, gb))) return rc;
+  for (long j = S - 1; j >= 0 && N > 0; --j) {
+    const long n0 = tp->snapAt[j];
+    long n1 = seg_end(j);
+    if (j == S - 1 && tp->lastIn && !tp->lastInSnap && N - 1 > n0) {
+      // the last step from its held input: where the schedule's first sweep
+      // ends (that sweep is the taped run itself)
+      if ((rc = reverse_step(s, tp, tp->lastIn, N - 1, gb))) return rc;
+      --n1;
+    }
+    if ((rc = reverse_range(s, tp, tp->snap[j], n0, n1 - n0, free_slots(j), gb))) return rc;
   }
   if ((rc = adjoint_outputs(s, tp, U0bar, theta_bar, mem, s->stream))) return rc;
   HIPCHK(s, hipStreamSynchronize(s->stream));

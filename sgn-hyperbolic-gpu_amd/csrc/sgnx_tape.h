@@ -15,8 +15,19 @@
 struct AdjTape {
   bool recording = false;
   long maxSnap = 0;
-  long K = 1;                      // snapshot stride in steps
-  std::vector<double*> snap;       // snap[j] = input state of step j*K
+  long K = 1;                      // snapshot stride in steps (doubling mode)
+  std::vector<double*> snap;       // snap[j] = input state of step snapAt[j]
+  std::vector<long> snapAt;        // ascending, snapAt[0] = 0
+  // planned mode (planStates > 0): snapshots at the steps of planAt, the
+  // first sweep of the binomial schedule for the planned step count; the
+  // sweep then holds at most planStates states (snapshots included)
+  long planStates = 0;
+  std::vector<long> planAt;
+  size_t nextPlan = 0;
+  // planned mode: the input state of the last accepted step (a snapshot, or
+  // held besides them), where the schedule's first sweep ends
+  double* lastIn = nullptr;
+  bool lastInSnap = false;
   bool pendingSnap = false;        // the step under way is to be snapshot on acceptance
   std::vector<double*> spare;      // allocated, unused snapshot buffers
   std::vector<double> t, dt;       // accepted steps

@@ -367,6 +367,13 @@ class SGN2D:
         integrateAdaptive calls (at most max_snapshots states held)."""
         L.check(self.lib.sgnx_tape_begin(self.h, int(max_snapshots)), self.h)
 
+    def tapeBeginPlan(self, steps, states):
+        """Record like tapeBegin, for a run of `steps` accepted steps whose
+        reverse sweep holds at most `states` states (snapshots included):
+        snapshots at the first sweep of the binomial schedule, so the sweep
+        recomputes the fewest steps that budget allows."""
+        L.check(self.lib.sgnx_tape_begin_plan(self.h, int(steps), int(states)), self.h)
+
     def tapeSlots(self, slots):
         """States the reverse sweep may hold besides the snapshots (-1: all a
         segment needs; fewer: binomial recomputation, same result)."""

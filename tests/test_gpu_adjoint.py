@@ -254,6 +254,52 @@ def test_checkpoint_budgets_give_the_same_adjoint(budget, slots):
     assert_bitwise(res[1][1], res[0][1], "thetaBar under a checkpoint budget")
 
 
+@pytest.mark.parametrize("plan,states,run", [(13, 3, 13), (13, 4, 13), (13, 6, 13), (13, 13, 13),
+                                              (13, 4, 9), (9, 3, 13)])
+def test_planned_tape_recomputes_the_fewest_steps(plan, states, run):
+    """A tape planned for `plan` steps with `states` held states (snapshots at
+    the binomial schedule's first sweep) gives the store-all cotangents bit for
+    bit. When the run takes the planned steps, the sweep recomputes exactly
+    F(L, states - 1) - (L - 1) steps, F the optimum of the dynamic programme
+    over the first checkpoint (the taped run is the schedule's first sweep);
+    a shorter or longer run is still reversed exactly."""
+    import sgnx
+    c = _cases()[2]
+    theta = c.depth["Hnod"].copy()
+    nodes = np.array([5, 77, 300], dtype=np.int64)
+    r = np.random.default_rng(11)
+    res = []
+    for planned in (False, True):
+        s = sgnx.SGN2D(**c.solver_kwargs())
+        s.setDepth(theta)
+        s.upload(c.U0)
+        s.setGauges(nodes, 4 * run)
+        dt0 = s.cflTimeStep(None, 0.5)
+        if planned:
+            s.tapeBeginPlan(plan, states)
+        else:
+            s.tapeBegin(1 << 20)
+        st = s.integrateAdaptive(0.0, 1e9, dt0,
+                                 sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=run))
+        s.tapeEnd()
+        assert st.accepted == run
+        UN, G = s.download(), s.gaugeLog()[1]
+        steps, stride, held = s.tapeInfo()
+        if planned:
+            assert steps == run and stride == 0 and held <= states
+        rr = np.random.default_rng(9)
+        res.append(s.adjoint(rr.normal(size=UN.size), rr.normal(size=G.shape)))
+        recomputed, peak = s.adjointStats()
+        if planned:
+            assert peak <= states - 1, (peak, states)
+            if run == plan:
+                want = _recompute_cost(plan, states - 1) - (plan - 1)
+                assert recomputed == want, (plan, states, recomputed, want)
+        s.close()
+    assert_bitwise(res[1][0], res[0][0], "U0bar of the planned tape")
+    assert_bitwise(res[1][1], res[0][1], "thetaBar of the planned tape")
+
+
 @pytest.mark.parametrize("ci", range(4))
 def test_kept_stages_give_the_same_adjoint(ci):
     """A taped run that keeps its stage inputs (written by the forward's own
