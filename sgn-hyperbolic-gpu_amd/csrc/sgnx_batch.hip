@@ -35,7 +35,7 @@ struct sgnx_batch {
   std::vector<sgnx_solver*> m;
   int n = 0;
   int device = 0;
-  int waves3 = 0x0;                    // bit k: launch kind k built for 3 waves/SIMD
+  int waves3 = 0x2;                    // bit k: launch kind k built for 3 waves/SIMD
   hipStream_t stream = nullptr;
   // argument tables, [4 launch kinds][n slots], and the tile offsets
   BatchMember* h_mem = nullptr;        // pinned

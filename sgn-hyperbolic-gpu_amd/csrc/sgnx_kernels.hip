@@ -589,14 +589,16 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
     return face_node(r, n);
   };
   // The interface lanes' global operands of row r + 1: the halo nodes and
-  // the bottom face of row r + 2. The stage launches issue them after their
-  // phase A of row r. The plain launch issues them at the start of phase C,
-  // which leaves phase C and the next phase A to cover their latency, brings
-  // the full loads of row r + 2 (issued after that phase A) close enough
-  // behind that fewer bottom-face lines have left L2, and shortens their
-  // register lifetime (measured 4 % faster; the stage launches were not
-  // faster this way).
-  constexpr bool kLateFace = MODE == kPlain;
+  // the bottom face of row r + 2. The two-wave stage launches issue them
+  // after their phase A of row r. The plain launch and the three-wave stage-2
+  // launch issue them at the start of phase C, which leaves phase C and the
+  // next phase A to cover their latency, brings the full loads of row r + 2
+  // (issued after that phase A) close enough behind that fewer bottom-face
+  // lines have left L2, and shortens their register lifetime: the plain
+  // launch measured 4 % faster, and the stage-2 build fits three waves per
+  // SIMD (with Hx, Hy read) with 3 registers spilled instead of 15. The
+  // two-wave stage launches were not faster this way.
+  constexpr bool kLateFace = MODE == kPlain || (MODE == kAxpy && WAVES == 3);
   StageRaw<MODE> fR;
   double fH = 0.0;
   auto face_prefetch = [&](int r) {
@@ -1221,9 +1223,10 @@ hipError_t launch_rhs_stage(const Geometry& g, const Phys& ph, int mode, const S
   // g.waves3 set: 3 waves, else 2)
   const bool w3 = (g.waves3 >> mode) & 1;
   if (mode < kPlain || mode > kErr || ph.nManning != 0.0) return hipErrorInvalidValue;
-  // forming Hx, Hy costs the plain build (three waves per SIMD) registers
-  // it does not have; the stage builds take it
-  const bool hxr = g.hxr != 0 && mode != kPlain;
+  // Hx, Hy formed from Hnod (when the device check allowed it) in the
+  // two-wave stage builds; the plain build and the three-wave stage-2 build
+  // have no registers for it and read them
+  const bool hxr = g.hxr != 0 && mode != kPlain && !(mode == kAxpy && w3);
 #define SGNX_SWEEP(M, W, K) sweep_launch<M, W, K>(hxr, grid, block, st, sa.u, dU, g, ph, gt, sa, strips)
   const bool keep = sa.yOut != nullptr;
   if (mode == kPlain) { if (w3) SGNX_SWEEP(kPlain, 3, false); else SGNX_SWEEP(kPlain, 2, false); }
@@ -1246,7 +1249,7 @@ hipError_t launch_rhs_stage_batch(const BatchMember* mem, const long long* tileS
   if (nMem < 1 || totalTiles < 1 || totalTiles > 0x7fffffffLL) return hipErrorInvalidValue;
   if (mode < kPlain || mode > kErr) return hipErrorInvalidValue;
   const dim3 grid((unsigned)totalTiles), block(Sweep<kTileW>::NT);
-  hxr = hxr && mode != kPlain;
+  hxr = hxr && mode != kPlain && !(mode == kAxpy && waves3);
 #define SGNX_SWEEPB(M, W, K) sweep_launch_batch<M, W, K>(hxr, grid, block, st, mem, tileStart, nMem)
   if (mode == kPlain) { if (waves3) SGNX_SWEEPB(kPlain, 3, false); else SGNX_SWEEPB(kPlain, 2, false); }
   else if (mode == kAxpy && keep) { if (waves3) SGNX_SWEEPB(kAxpy, 3, true); else SGNX_SWEEPB(kAxpy, 2, true); }
