@@ -215,3 +215,40 @@ def test_rhs_device_pointers_and_determinism():
     assert_bitwise(a, want, "device-pointer RHS")
     for d in (U, d1, d2):
         d.free()
+
+
+@pytest.mark.parametrize("ci", range(3))
+def test_wave_and_gradient_builds_give_the_same_bits(ci, monkeypatch):
+    """The builds a launch kind can take (two or three waves per SIMD, Hx, Hy
+    read or formed from Hnod in phase C, the face operands issued after
+    phase A or at the start of phase C) are the same arithmetic: 12 adaptive
+    steps under every wave mask and either gradient setting give the oracle's
+    step counts and one final state bit for bit, and the plain RHS of that
+    state is the oracle's."""
+    import sgnx
+    from sgnx import cases
+    case = [cases.gate(Nx=11, Ny=7, bc=("periodic", "periodic", "wall", "wall")),
+            cases.solitary(Nx=48, Ny=24, walls=True),
+            cases.shoal(seed=3, Nx=40, Ny=20)][ci]
+    o = OL.Oracle.from_case(case)
+    dt0 = o.cfl_time_step(case.U0, 0.5)
+    opts = dict(atol=1e-8, rtol=1e-6, max_accepted=12)
+    Uo, sto, *_ = o.integrate(case.U0, 0.0, 1e9, dt0, OL.OrcOpts(**opts))
+    finals = []
+    for mask in (0x0, 0x2, 0xF, 0x1, 0xA):
+        for hxr in ("1", "0"):
+            monkeypatch.setenv("SGNX_SWEEP_WAVES3", str(mask))
+            monkeypatch.setenv("SGNX_SWEEP_HXR", hxr)
+            s = sgnx.SGN2D(**case.solver_kwargs())
+            s.upload(case.U0)
+            st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(**opts))
+            assert (st.accepted, st.rejected) == (sto.accepted, sto.rejected), (mask, hxr)
+            U = s.download()
+            finals.append(((mask, hxr), U))
+            assert_bitwise(s.rhs(U), o.rhs(U), f"{case.name} plain RHS, waves {mask:#x} hxr {hxr}")
+            s.close()
+    o.close()
+    for key, U in finals[1:]:
+        assert_bitwise(U, finals[0][1], f"{case.name} state after 12 steps, {key}")
+    rel = np.linalg.norm(finals[0][1] - Uo) / np.linalg.norm(Uo)
+    assert rel <= 1e-12, rel
