@@ -276,6 +276,10 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots);
  * snapshots; the steps of a longer run past the plan form one segment. No
  * reference counterpart (the reference has no adjoint). */
 int sgnx_tape_begin_plan(sgnx_solver* s, long steps, long states);
+/* The snapshot steps sgnx_tape_begin_plan takes for (steps, states), in
+ * ascending order from 0 (up to cap of them into at; n = their count). Host
+ * only, no device needed. */
+int sgnx_tape_plan_positions(long steps, long states, long* at, long cap, long* n);
 /* States the reverse sweep may hold besides the snapshots while it recomputes
  * a segment (-1, the default: all K - 1 of it). Fewer slots select a binomial
  * schedule (Griewank's revolve bound) inside each segment, plus two states of

@@ -610,6 +610,14 @@ int sgnx_tape_begin_plan(sgnx_solver* s, long steps, long states) {
   return SGNX_OK;
 }
 
+int sgnx_tape_plan_positions(long steps, long states, long* at, long cap, long* n) {
+  if (!n || steps < 1 || states < 1 || cap < 0 || (cap > 0 && !at)) return SGNX_EINVAL;
+  const std::vector<long> p = plan_positions(steps, states);
+  for (long k = 0; k < std::min((long)p.size(), cap); ++k) at[k] = p[k];
+  *n = (long)p.size();
+  return SGNX_OK;
+}
+
 int sgnx_tape_set_slots(sgnx_solver* s, long slots) {
   if (!s) return SGNX_EINVAL;
   if (slots < -1) return fail(s, SGNX_EINVAL, "tape: slots must be >= -1");

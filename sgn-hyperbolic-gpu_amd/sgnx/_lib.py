@@ -121,6 +121,8 @@ SIGNATURES = [
                                C.c_size_t, C.c_double, C.c_int]),
     ("sgnx_tape_begin", C.c_int, [C.c_void_p, C.c_long]),
     ("sgnx_tape_begin_plan", C.c_int, [C.c_void_p, C.c_long, C.c_long]),
+    ("sgnx_tape_plan_positions", C.c_int, [C.c_long, C.c_long, C.POINTER(C.c_long), C.c_long,
+                                           C.POINTER(C.c_long)]),
     ("sgnx_tape_end", C.c_int, [C.c_void_p]),
     ("sgnx_tape_set_slots", C.c_int, [C.c_void_p, C.c_long]),
     ("sgnx_tape_keep_stages", C.c_int, [C.c_void_p, C.c_int]),

@@ -113,3 +113,48 @@ def test_no_device_is_an_error_not_an_abort():
                          timeout=120)
     assert out.returncode == 0, out.stderr
     assert "ERR 2" in out.stdout or "ERR 3" in out.stdout, out.stdout + out.stderr
+
+
+def _revolve_cost(L, c, memo={}):
+    """Forward steps an optimal schedule needs to reverse L steps from a held
+    state with c free state slots (dynamic programme over the first checkpoint)."""
+    if L <= 1:
+        return 0
+    if c <= 0:
+        return L * (L - 1) // 2
+    key = (L, c)
+    if key not in memo:
+        memo[key] = min(m + _revolve_cost(L - m, c - 1) + _revolve_cost(m, c) for m in range(1, L))
+    return memo[key]
+
+
+@pytest.mark.parametrize("steps,states", [(13, 3), (13, 4), (13, 6), (13, 13), (40, 5), (60, 7),
+                                          (97, 9), (2, 1), (1, 3), (30, 1)])
+def test_planned_snapshots_give_the_binomial_optimum(sgnx_mod, steps, states):
+    """sgnx_tape_plan_positions (host only): snapshot steps from 0, ascending,
+    at most `states` of them; reversing segment j with states - j - 1 free
+    slots, the last step's input held from the taped run, recomputes exactly
+    F(L, states - 1) - (L - 1) steps, F the dynamic programme's optimum."""
+    lib = sgnx_mod.load()
+    at = (C.c_long * 64)()
+    n = C.c_long()
+    assert lib.sgnx_tape_plan_positions(steps, states, at, 64, C.byref(n)) == 0
+    p = list(at[:n.value])
+    assert p[0] == 0 and all(a < b for a, b in zip(p, p[1:])) and p[-1] < steps
+    assert len(p) <= states
+    c = states - 1
+    cost = 0
+    for j, a in enumerate(p):
+        b = p[j + 1] if j + 1 < len(p) else steps
+        if j == len(p) - 1 and b - a >= 2:
+            b -= 1                       # the last step's input is held: reversed without recomputation
+        cost += _revolve_cost(b - a, c - j)
+    assert cost == _revolve_cost(steps, c) - (steps - 1), (p, cost)
+
+
+def test_plan_positions_rejects_bad_arguments(sgnx_mod):
+    lib = sgnx_mod.load()
+    n = C.c_long()
+    assert lib.sgnx_tape_plan_positions(0, 3, None, 0, C.byref(n)) != 0
+    assert lib.sgnx_tape_plan_positions(10, 0, None, 0, C.byref(n)) != 0
+    assert lib.sgnx_tape_plan_positions(10, 3, None, 0, C.byref(n)) == 0 and n.value >= 1
