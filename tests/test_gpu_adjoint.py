@@ -662,6 +662,47 @@ def test_batched_forward_records_the_solo_tapes(keep):
         s.close()
 
 
+def test_batched_forward_records_planned_tapes():
+    """Members whose tapes are planned (binomial snapshots for their step
+    counts, fewer states than steps) record them inside a batched adaptive run
+    (graph on) as they would solo, and each member's own reverse sweep gives
+    its store-all cotangents bit for bit."""
+    import sgnx
+    from sgnx import cases
+    cs = [cases.gate(Nx=8, Ny=6), cases.solitary(Nx=32, Ny=16), cases.shoal(seed=3, Nx=48, Ny=24)]
+    steps, states = [9, 7, 8], [3, 4, 2]
+    rng = np.random.default_rng(7)
+    want, ss, cot = [], [], []
+    for c, k in zip(cs, steps):
+        nodes = rng.choice(c.nd, size=3, replace=False).astype(np.int64)
+        s, UN, G = _taped_run(c, c.depth["Hnod"], c.U0, nodes, k, 1 << 20)
+        ub, gb = rng.normal(size=UN.size), rng.normal(size=G.shape)
+        want.append(s.adjoint(ub, gb))
+        cot.append((ub, gb, nodes))
+        ss.append(s)
+    for c, s, (_, _, nodes), k, m in zip(cs, ss, cot, steps, states):
+        s.upload(c.U0)
+        s.setGauges(nodes, 64)
+        s.tapeBeginPlan(k, m)
+    b = sgnx.Batch(ss)
+    dt0 = [s.cflTimeStep(None, 0.5) for s in ss]
+    for c, s in zip(cs, ss):
+        s.upload(c.U0)
+    opts = [sgnx.RkOpts(atol=1e-8, rtol=1e-6, max_accepted=k) for k in steps]
+    b.integrateAdaptive(0.0, 1e9, dt0, opts, graph=True)
+    for s in ss:
+        s.tapeEnd()
+    b.close()
+    for i, (s, (ub, gb, _)) in enumerate(zip(ss, cot)):
+        n, stride, held = s.tapeInfo()
+        assert n == steps[i] and stride == 0 and held <= states[i]
+        U0b, thb = s.adjoint(ub, gb)
+        assert s.adjointStats()[1] <= states[i] - 1
+        assert_bitwise(U0b, want[i][0], f"member {i} U0bar (planned tape)")
+        assert_bitwise(thb, want[i][1], f"member {i} thetaBar (planned tape)")
+        s.close()
+
+
 def test_batched_adjoint_of_large_members_matches_solo_bitwise():
     """Members that each fill the GPU (at least six product workgroups per
     compute unit) run their own sweeps inside the batched call: every
