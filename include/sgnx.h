@@ -273,8 +273,9 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots);
  * that budget, and the sweep reverses segment j with states - j - 1 slots,
  * which recomputes the fewest steps the budget allows (sgnx_tape_set_slots is
  * not used). A run of fewer steps is reversed correctly with fewer
- * snapshots; the steps of a longer run past the plan form one segment. No
- * reference counterpart (the reference has no adjoint). */
+ * snapshots; the steps of a longer run past the plan form one segment.
+ * steps <= 2^20 (SGNX_EINVAL beyond). No reference counterpart (the reference
+ * has no adjoint). */
 int sgnx_tape_begin_plan(sgnx_solver* s, long steps, long states);
 /* The snapshot steps sgnx_tape_begin_plan takes for (steps, states), in
  * ascending order from 0 (up to cap of them into at; n = their count). Host

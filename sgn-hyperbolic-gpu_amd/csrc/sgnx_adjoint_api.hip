@@ -178,12 +178,16 @@ int ensure_vjp_buffers(sgnx_solver* s, AdjTape* tp) {
 }
 
 // ---- recomputation inside a snapshot segment ------------------------------------
+// the largest step count a tape plan covers
+constexpr long kMaxPlanSteps = 1L << 20;
+
 // F(L, c): forward steps needed to reverse L steps from a held start state
 // with c free state slots, F(1, c) = 0, F(L, 0) = L (L - 1) / 2, and
 // F(L, c) = min_m m + F(L - m, c - 1) + F(m, c): Griewank's binomial bound
 // r L - binom(C + r, C + 1) with C = c + 1 checkpoints (the start state's
 // included) and r the least repetition number with binom(C + r, C) >= L.
 double binom_capped(long n, long k) {
+  k = std::min(k, n - k);  // binom(n, k) = binom(n, n - k): the shorter product
   double r = 1.0;
   for (long i = 1; i <= k; ++i) {
     r = r * (double)(n - k + i) / (double)i;
@@ -201,17 +205,20 @@ long fwd_cost(long L, long c) {
   return r * L - (long)binom_capped(C + r, C + 1);
 }
 
-// the first checkpoint of an optimal schedule: advance m steps and keep the state
+// the first checkpoint of an optimal schedule: advance m steps and keep the
+// state. g(m) = m + F(L - m, c - 1) + F(m, c) is convex in m (F(., c) has
+// nondecreasing increments, the repetition numbers), so its smallest
+// minimiser is the first m whose forward difference is not negative: a
+// binary search, the same m a scan over 1 .. L - 1 picks first.
 long best_split(long L, long c) {
-  long best = 1, cost = -1;
-  for (long m = 1; m < L; ++m) {
-    const long k = m + fwd_cost(L - m, c - 1) + fwd_cost(m, c);
-    if (cost < 0 || k < cost) {
-      cost = k;
-      best = m;
-    }
+  auto g = [&](long m) { return m + fwd_cost(L - m, c - 1) + fwd_cost(m, c); };
+  long lo = 1, hi = L - 1;
+  while (lo < hi) {
+    const long mid = lo + (hi - lo) / 2;
+    if (g(mid + 1) - g(mid) >= 0) hi = mid;
+    else lo = mid + 1;
   }
-  return best;
+  return lo;
 }
 
 // The snapshot steps of a planned tape: the first forward sweep of the
@@ -597,8 +604,8 @@ int sgnx_tape_begin(sgnx_solver* s, long max_snapshots) {
 
 int sgnx_tape_begin_plan(sgnx_solver* s, long steps, long states) {
   if (!s) return SGNX_EINVAL;
-  if (steps < 1 || states < 1)
-    return fail(s, SGNX_EINVAL, "tape: the plan needs steps >= 1 and states >= 1");
+  if (steps < 1 || states < 1 || steps > kMaxPlanSteps)
+    return fail(s, SGNX_EINVAL, "tape: the plan needs 1 <= steps <= 2^20 and states >= 1");
   std::vector<long> at = plan_positions(steps, states);
   int rc;
   if ((rc = sgnx_tape_begin(s, (long)at.size()))) return rc;
@@ -611,7 +618,8 @@ int sgnx_tape_begin_plan(sgnx_solver* s, long steps, long states) {
 }
 
 int sgnx_tape_plan_positions(long steps, long states, long* at, long cap, long* n) {
-  if (!n || steps < 1 || states < 1 || cap < 0 || (cap > 0 && !at)) return SGNX_EINVAL;
+  if (!n || steps < 1 || steps > kMaxPlanSteps || states < 1 || cap < 0 || (cap > 0 && !at))
+    return SGNX_EINVAL;
   const std::vector<long> p = plan_positions(steps, states);
   for (long k = 0; k < std::min((long)p.size(), cap); ++k) at[k] = p[k];
   *n = (long)p.size();

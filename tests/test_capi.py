@@ -158,3 +158,7 @@ def test_plan_positions_rejects_bad_arguments(sgnx_mod):
     assert lib.sgnx_tape_plan_positions(0, 3, None, 0, C.byref(n)) != 0
     assert lib.sgnx_tape_plan_positions(10, 0, None, 0, C.byref(n)) != 0
     assert lib.sgnx_tape_plan_positions(10, 3, None, 0, C.byref(n)) == 0 and n.value >= 1
+    assert lib.sgnx_tape_plan_positions((1 << 20) + 1, 40, None, 0, C.byref(n)) != 0
+    at = (C.c_long * 64)()
+    assert lib.sgnx_tape_plan_positions(1 << 20, 40, at, 64, C.byref(n)) == 0
+    assert 1 <= n.value <= 40 and at[0] == 0
