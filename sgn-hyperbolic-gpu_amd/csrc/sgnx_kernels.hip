@@ -844,11 +844,17 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
         for (int v = 0; v < NV; ++v) variable(v);
       } else {
         // unrolled, so that the per-node operands stay in registers without
-        // selects; the scheduling fence keeps each variable's LDS operands
-        // from being hoisted into the previous one
+        // selects. Scheduling fences bound how far the LDS operands of later
+        // variables are hoisted: the plain and stage-2 builds take one
+        // variable per group; the stage-3 and k4 builds two groups, (h, hu)
+        // and (hv, hw, hp), which lets one variable's LDS reads overlap
+        // another's arithmetic within their two-wave register budget (k4 2 %
+        // and stage 3 1 % faster, same bits; grouped so, the plain build ran
+        // slower)
+        constexpr bool kGrouped = MODE == kCand || MODE == kErr;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          __builtin_amdgcn_sched_barrier(0);
+          if (!kGrouped || v == 0 || v == 2) __builtin_amdgcn_sched_barrier(0);
           variable(v);
         }
       }
