@@ -845,13 +845,12 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
       } else {
         // unrolled, so that the per-node operands stay in registers without
         // selects. Scheduling fences bound how far the LDS operands of later
-        // variables are hoisted: the plain and stage-2 builds take one
-        // variable per group; the stage-3 and k4 builds two groups, (h, hu)
+        // variables are hoisted: the stage builds take two groups, (h, hu)
         // and (hv, hw, hp), which lets one variable's LDS reads overlap
-        // another's arithmetic within their two-wave register budget (k4 2 %
-        // and stage 3 1 % faster, same bits; grouped so, the plain build ran
-        // slower)
-        constexpr bool kGrouped = MODE == kCand || MODE == kErr;
+        // another's arithmetic within their two-wave register budget (k4 2 %,
+        // stage 3 1 %, the two-wave stage 2 2.4 % faster, same bits); the
+        // plain build keeps one variable per group (grouped, it ran slower)
+        constexpr bool kGrouped = MODE != kPlain;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           if (!kGrouped || v == 0 || v == 2) __builtin_amdgcn_sched_barrier(0);
