@@ -289,9 +289,9 @@ __device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry&
   const double* sgR = L.gR;
   const double iJx = 1.0 / g.Jx, iJy = 1.0 / g.Jy;
 
-  // phase 1: node loads, the cotangents of the two D-sums and of the depth
-  // gradients of the non-conservative products; the face threads (20 per
-  // element, see phase 2) issue their global loads before the barrier too
+  // phase 1: the cotangent loads, the cotangents of the two D-sums and of the
+  // depth gradients of the non-conservative products; the face threads (20
+  // per element, see phase 2) issue their global loads before the barrier too
   const int j = t / NPP, loc = t - NPP * (t / NPP);
   const int b = loc / NP, a = loc - NP * b;
   const bool isNode = j < nW;
@@ -347,15 +347,6 @@ __device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry&
                            : g.HfY[((long long)line * Nx + ex) * NP + n];
       }
     }
-  }
-  // the node's own operands of phase 3
-  double Un[NV], Hl = 0.0, Hxl = 0.0, Hyl = 0.0;
-  if (isNode) {
-#pragma unroll
-    for (int c = 0; c < NV; ++c) Un[c] = U[(long long)c * Nd + i];
-    Hl = g.Hnod[i];
-    Hxl = g.Hx[i];
-    Hyl = g.Hy[i];
   }
   tile_stage(va, g, ph, L, t, isNode, i, sH);
   __syncthreads();
@@ -419,6 +410,17 @@ __device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry&
   }
   __syncthreads();
 
+  // the node's own operands of phase 3, loaded after the face phase (h, hu,
+  // hv are lines tile_stage has just read), which keeps them out of the
+  // registers live across it: 0.7 % faster than loading them in phase 1
+  double Un[NV], Hl = 0.0, Hxl = 0.0, Hyl = 0.0;
+  if (isNode) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) Un[c] = U[(long long)c * Nd + i];
+    Hl = g.Hnod[i];
+    Hxl = g.Hx[i];
+    Hyl = g.Hy[i];
+  }
   // phase 3: node adjoint
   const int jb = NPP * j, fb = 20 * j;
   double Hxb = 0.0, Hyb = 0.0, th = 0.0;
