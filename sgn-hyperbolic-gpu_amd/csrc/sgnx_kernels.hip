@@ -845,15 +845,16 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
       } else {
         // unrolled, so that the per-node operands stay in registers without
         // selects. Scheduling fences bound how far the LDS operands of later
-        // variables are hoisted: the stage builds take two groups, (h, hu)
-        // and (hv, hw, hp), which lets one variable's LDS reads overlap
-        // another's arithmetic within their two-wave register budget (k4 2 %,
-        // stage 3 1 %, the two-wave stage 2 2.4 % faster, same bits); the
-        // plain build keeps one variable per group (grouped, it ran slower)
+        // variables are hoisted: the stage builds take two groups, (the depth
+        // gradient, h, hu) and (hv, hw, hp), which lets one variable's LDS
+        // reads overlap another's arithmetic within their two-wave register
+        // budget (k4 2.5 %, stage 3 1.5 %, the two-wave stage 2 2.4 % faster,
+        // same bits); the plain build keeps the depth gradient and each
+        // variable in a group of its own (grouped, it ran slower)
         constexpr bool kGrouped = MODE != kPlain;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-          if (!kGrouped || v == 0 || v == 2) __builtin_amdgcn_sched_barrier(0);
+          if (kGrouped ? v == 2 : true) __builtin_amdgcn_sched_barrier(0);
           variable(v);
         }
       }
