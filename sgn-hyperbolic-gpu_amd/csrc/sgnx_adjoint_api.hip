@@ -7,7 +7,10 @@
 // gauge log row) and snapshots of the step input states: every K-th step,
 // with K doubled (every other snapshot dropped) whenever the snapshot budget
 // is full, so at most `max_snapshots` states are held whatever the run's
-// length. The reverse sweep walks the snapshot segments backwards; inside a
+// length; or, for a run whose step count is given in advance (a planned
+// tape), at the steps of the binomial schedule's first forward sweep, so that
+// the reversal recomputes the fewest steps `states` states allow (Griewank's
+// bound). The reverse sweep walks the snapshot segments backwards; inside a
 // segment it recomputes the states with the forward's own stage kernels
 // (bitwise the recorded trajectory), holding at most `slots` of them at once
 // (a binomial schedule, Griewank's revolve bound, when that is fewer than the
