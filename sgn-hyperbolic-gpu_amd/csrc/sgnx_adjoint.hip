@@ -427,7 +427,9 @@ __device__ __forceinline__ void vjp_elem_tile(const VjpArgs& va, const Geometry&
   double outv[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};   // the node's result, for the next product
   if (isNode) {
     double dhx = 0.0, dhy = 0.0;
-#pragma unroll
+    // rolled, like the D^T loop below: unrolled, the compiler hoists its LDS
+    // operands into 20 more VGPRs, which keeps the kernel off four waves
+#pragma unroll 1
     for (int k = 0; k < NP; ++k) {
       dhx = dhx + sD[a * NP + k] * sH[jb + k + NP * b];
       dhy = dhy + sD[b * NP + k] * sH[jb + a + NP * k];
@@ -546,7 +548,10 @@ k_vjp_coef(VjpArgs va, Geometry g, Phys ph) {
   vjp_coef_tile(va, g, ph, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kVT)
+// four waves per SIMD (at most 128 VGPRs; 5 spilled, 20 B of scratch per
+// lane): 5.6 % faster per product and 8.7 % per reversed step than three
+// waves (134 VGPRs), same bits
+__global__ void __launch_bounds__(kVT, 4)
 k_vjp_elem(VjpArgs va, Geometry g, Phys ph, GhostTime gt) {
   vjp_elem_tile(va, g, ph, gt, blockIdx.x);
 }
@@ -571,7 +576,7 @@ k_vjp_coef_batch(const VjpMember* __restrict__ mem, const long long* __restrict_
   vjp_coef_tile(mem[m].va, mem[m].g, mem[m].ph, (int)(w - tileStart[m]));
 }
 
-__global__ void __launch_bounds__(kVT)
+__global__ void __launch_bounds__(kVT, 4)
 k_vjp_elem_batch(const VjpMember* __restrict__ mem, const long long* __restrict__ tileStart,
                  int nMem) {
   const long long w = blockIdx.x;
