@@ -786,7 +786,11 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
       const int bot = r & 1, top = (r + 1) & 1;
       const int rowx = j * NPP + NP * b, coly = j * NPP + a;
       double dhx = 0.0, dhy = 0.0, dHx = 0.0, dHy = 0.0;
-#pragma unroll
+      // the three-wave stage-2 build keeps this loop rolled: unrolled, its
+      // hoisted LDS operands cost the 3 registers it otherwise spills (1 %
+      // faster without the spill; rolled, the other builds ran 3-8 % slower)
+      constexpr int kDepthUnroll = (WAVES == 3 && MODE == kAxpy) ? 1 : NP;
+#pragma unroll kDepthUnroll
       for (int k = 0; k < NP; ++k) {
         dhx = dhx + Da[k] * sU[IH][rowx + k];
         dhy = dhy + Db[k] * sU[IH][coly + NP * k];
