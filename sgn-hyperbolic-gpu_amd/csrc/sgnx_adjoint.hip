@@ -203,7 +203,9 @@ __device__ __forceinline__ void face_coef(const TileLds& L, int jb, int f, int n
   const bool hi = (f & 1) != 0;
   double sum[NV] = {0.0, 0.0, 0.0, 0.0, 0.0};
   double q = 0.0;
-#pragma unroll
+  // rolled: fewer registers live in the product kernel's face phase (its
+  // four-wave build spills 3 registers instead of 5)
+#pragma unroll 1
   for (int k = 0; k < NP; ++k) {
     const int node = jb + (dir == 0 ? k + NP * n : n + NP * k);
     const double gk = hi ? L.gR[k] : L.gL[k];
@@ -548,7 +550,7 @@ k_vjp_coef(VjpArgs va, Geometry g, Phys ph) {
   vjp_coef_tile(va, g, ph, blockIdx.x);
 }
 
-// four waves per SIMD (at most 128 VGPRs; 5 spilled, 20 B of scratch per
+// four waves per SIMD (at most 128 VGPRs; 3 spilled, 12 B of scratch per
 // lane): 5.6 % faster per product and 8.7 % per reversed step than three
 // waves (134 VGPRs), same bits
 __global__ void __launch_bounds__(kVT, 4)
