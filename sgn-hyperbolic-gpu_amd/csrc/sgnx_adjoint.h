@@ -50,14 +50,6 @@ struct VjpMember {
   Phys ph;
   GhostTime gt;
 };
-// ... a stage combination out = u + a k per member ...
-struct CombMember {
-  double* out;
-  const double* u;
-  double a;
-  const double* k;
-  long long n;
-};
 // ... and one gauge log row's cotangents per member (Gb null: none)
 struct GaugeMember {
   const double* Gb;
@@ -67,11 +59,8 @@ struct GaugeMember {
   double* thetaBar;
 };
 long long vjp_tiles(const Geometry& g);       // workgroups of one product launch
-long long comb_blocks(long long n);           // workgroups of a combination of n doubles
 hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart, int nMem,
                                 long long totalTiles, bool coefReady, hipStream_t st);
-hipError_t launch_comb_batch(const CombMember* mem, const long long* blockStart, int nMem,
-                             long long totalBlocks, hipStream_t st);
 hipError_t launch_gauge_bar_batch(const GaugeMember* gm, int nMem, hipStream_t st);
 
 hipError_t launch_rhs_vjp(const Geometry& g, const Phys& ph, const VjpArgs& va, GhostTime gt,

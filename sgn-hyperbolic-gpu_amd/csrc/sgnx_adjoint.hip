@@ -586,22 +586,6 @@ k_vjp_elem_batch(const VjpMember* __restrict__ mem, const long long* __restrict_
   vjp_elem_tile(mem[m].va, mem[m].g, mem[m].ph, mem[m].gt, (int)(w - tileStart[m]));
 }
 
-// out = u + a k over every member's n doubles; blockStart over chunks of kT * 4
-constexpr int kCombChunk = kT * 4;
-__global__ void __launch_bounds__(kT)
-k_comb_batch(const CombMember* __restrict__ mem, const long long* __restrict__ blockStart,
-             int nMem) {
-  const long long w = blockIdx.x;
-  const int m = member_of(blockStart, nMem, w);
-  const CombMember c = mem[m];
-  const long long base = (w - blockStart[m]) * kCombChunk;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const long long i = base + r * kT + threadIdx.x;
-    if (i < c.n) c.out[i] = c.u[i] + c.a * c.k[i];
-  }
-}
-
 // the gauge cotangents of one log row per member (one block per member, one
 // thread, gauge order)
 __global__ void k_gauge_bar_batch(const GaugeMember* __restrict__ gm, int nMem) {
@@ -674,8 +658,6 @@ hipError_t launch_rhs_vjp(const Geometry& g, const Phys& ph, const VjpArgs& va, 
 
 long long vjp_tiles(const Geometry& g) { return (long long)g.Ny * ((g.Nx + kEW - 1) / kEW); }
 
-long long comb_blocks(long long n) { return (n + kCombChunk - 1) / kCombChunk; }
-
 hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart, int nMem,
                                 long long totalTiles, bool coefReady, hipStream_t st) {
   if (totalTiles <= 0) return hipSuccess;
@@ -687,14 +669,6 @@ hipError_t launch_rhs_vjp_batch(const VjpMember* mem, const long long* tileStart
   }
   hipLaunchKernelGGL(k_vjp_elem_batch, dim3((unsigned)totalTiles), dim3(kVT), 0, st, mem,
                      tileStart, nMem);
-  return hipGetLastError();
-}
-
-hipError_t launch_comb_batch(const CombMember* mem, const long long* blockStart, int nMem,
-                             long long totalBlocks, hipStream_t st) {
-  if (totalBlocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_comb_batch, dim3((unsigned)totalBlocks), dim3(kT), 0, st, mem, blockStart,
-                     nMem);
   return hipGetLastError();
 }
 

@@ -156,19 +156,22 @@ int step_adjoint(sgnx_solver* s, AdjTape* tp, const double* u, double t, double 
   double* y3 = s->d_k3;
   HIPCHK(s, launch_rhs(s->g, s->ph, u, s->d_k1, ghost_time(s, t), s->stream));
   if (s->g.variant == 1) {
-    // the forward's stage-2 launch, keeping its input: y2 and k2 = R(y2) in one pass
+    // the forward's stage-2 launch, keeping its input and forming the next
+    // one: y2, k2 = R(y2) and y3 in one pass
     Stage a{};
     a.u = u;
     a.k1 = s->d_k1;
     a.a = 0.5 * dt;
     a.yOut = y2;
+    a.y3Out = y3;
+    a.a3 = 0.75 * dt;
     HIPCHK(s, launch_rhs_stage(s->g, s->ph, kAxpy, a, s->d_k2, ghost_time(s, t + 0.5 * dt),
                                s->stream));
   } else {
     HIPCHK(s, launch_comb1(y2, u, 0.5 * dt, s->d_k1, n, s->stream));
     HIPCHK(s, launch_rhs(s->g, s->ph, y2, s->d_k2, ghost_time(s, t + 0.5 * dt), s->stream));
+    HIPCHK(s, launch_comb1(y3, u, 0.75 * dt, s->d_k2, n, s->stream));
   }
-  HIPCHK(s, launch_comb1(y3, u, 0.75 * dt, s->d_k2, n, s->stream));
   return step_adjoint_stored(s, tp, u, y2, y3, t, dt, thetaBar);
 }
 

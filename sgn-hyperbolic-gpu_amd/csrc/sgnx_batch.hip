@@ -575,14 +575,12 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
     Nmax = std::max(Nmax, (long)tp[i]->t.size());
   }
   // argument arena: gauges, two RHS tables (R(u); the stage-2 launch keeping
-  // y2) + offsets, the y3 combination + offsets, three products + offsets
+  // y2 and forming y3) + offsets, three products + offsets
   auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t nn = (size_t)n;
   const size_t oG = 0, oR1 = up(oG + nn * sizeof(GaugeMember));
   const size_t oR2 = up(oR1 + nn * sizeof(BatchMember)), oRT = up(oR2 + nn * sizeof(BatchMember));
-  const size_t oC3 = up(oRT + (nn + 1) * sizeof(long long));
-  const size_t oCT = up(oC3 + nn * sizeof(CombMember));
-  const size_t oV3 = up(oCT + (nn + 1) * sizeof(long long));
+  const size_t oV3 = up(oRT + (nn + 1) * sizeof(long long));
   const size_t oV2 = up(oV3 + nn * sizeof(VjpMember)), oV1 = up(oV2 + nn * sizeof(VjpMember));
   const size_t oVT = up(oV1 + nn * sizeof(VjpMember));
   const size_t total = up(oVT + (nn + 1) * sizeof(long long));
@@ -600,8 +598,6 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
   auto* hR1 = reinterpret_cast<BatchMember*>(b->h_adj + oR1);
   auto* hR2 = reinterpret_cast<BatchMember*>(b->h_adj + oR2);
   auto* hRT = reinterpret_cast<long long*>(b->h_adj + oRT);
-  auto* hC3 = reinterpret_cast<CombMember*>(b->h_adj + oC3);
-  auto* hCT = reinterpret_cast<long long*>(b->h_adj + oCT);
   auto* hV3 = reinterpret_cast<VjpMember*>(b->h_adj + oV3);
   auto* hV2 = reinterpret_cast<VjpMember*>(b->h_adj + oV2);
   auto* hV1 = reinterpret_cast<VjpMember*>(b->h_adj + oV1);
@@ -609,7 +605,7 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
   auto dptr = [&](size_t off) { return b->d_adj + off; };
   for (long k = 0; k < Nmax; ++k) {
     int A = 0, Rn = 0;   // members in this step; of those, the ones that recompute stages
-    long long rAcc = 0, cAcc = 0, vAcc = 0;
+    long long rAcc = 0, vAcc = 0;
     for (int i = 0; i < n; ++i) {
       AdjTape* T = tp[i];
       const long m = (long)T->t.size() - 1 - k;
@@ -617,7 +613,6 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
       sgnx_solver* s = b->m[i];
       const double t = T->t[m], dt = T->dt[m];
       const double* u = T->snap[m];
-      const long long nst = (long long)s->n();
       GaugeMember gq{};
       if (gb[i] && T->row[m] >= 0 && T->row[m] < s->grows) {
         gq.Gb = gb[i] + (size_t)T->row[m] * s->nG;
@@ -635,18 +630,18 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
         Stage s1{};
         s1.u = u;
         hR1[Rn] = member_args(s, u, s->d_k1, s1, t);
-        // the forward's stage-2 launch, keeping its input: y2 and k2 = R(y2)
+        // the forward's stage-2 launch, keeping its input and forming the
+        // next one: y2, k2 = R(y2) and y3 = u + 3dt/4 k2
         Stage s2{};
         s2.u = u;
         s2.k1 = s->d_k1;
         s2.a = 0.5 * dt;
         s2.yOut = s->d_tmp;
+        s2.y3Out = s->d_k3;
+        s2.a3 = 0.75 * dt;
         hR2[Rn] = member_args(s, u, s->d_k2, s2, t + 0.5 * dt);
         hRT[Rn] = rAcc;
         rAcc += sweep_tiles(s->g);
-        hC3[Rn] = CombMember{s->d_k3, u, 0.75 * dt, s->d_k2, nst};
-        hCT[Rn] = cAcc;
-        cAcc += comb_blocks(nst);
         ++Rn;
       }
       // uNew = u + dt ((2/9 k1 + 1/3 k2) + 4/9 k3)   (rk.cpp:106-108)
@@ -682,13 +677,11 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
       ++A;
     }
     hRT[Rn] = rAcc;
-    hCT[Rn] = cAcc;
     hVT[A] = vAcc;
     BCHK(b, hipMemcpyAsync(b->d_adj, b->h_adj, total, hipMemcpyHostToDevice, b->stream));
     const auto* dR1 = reinterpret_cast<const BatchMember*>(dptr(oR1));
     const auto* dR2 = reinterpret_cast<const BatchMember*>(dptr(oR2));
     const auto* dRT = reinterpret_cast<const long long*>(dptr(oRT));
-    const auto* dCT = reinterpret_cast<const long long*>(dptr(oCT));
     const auto* dVT = reinterpret_cast<const long long*>(dptr(oVT));
     BCHK(b, launch_gauge_bar_batch(reinterpret_cast<const GaugeMember*>(dptr(oG)), A, b->stream));
     if (Rn > 0) {
@@ -696,8 +689,6 @@ int sgnx_batch_adjoint(sgnx_batch* b, const double* const* Ubar_final,
                                      batch_hxr(b)));
       BCHK(b, launch_rhs_stage_batch(dR2, dRT, Rn, rAcc, kAxpy, (b->waves3 >> 1) & 1, b->stream,
                                      true, batch_hxr(b)));
-      BCHK(b, launch_comb_batch(reinterpret_cast<const CombMember*>(dptr(oC3)), dCT, Rn, cAcc,
-                                b->stream));
     }
     // the products chained: the first launches the coefficient kernel, each
     // later one finds its coefficients written by the one before

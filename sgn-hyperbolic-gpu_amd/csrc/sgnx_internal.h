@@ -93,6 +93,8 @@ struct Stage {
   double atol, rtol;  // kErr
   double* part;       // kErr
   double* yOut;       // kAxpy, kCand (optional): the stage input u + a*k1 out
+  double* y3Out;      // kAxpy with yOut (optional): the next stage's input u + a3*R out
+  double a3;          // its coefficient
 };
 
 // One grid of a batched launch: its operands, geometry, constants, inflow

@@ -829,6 +829,11 @@ __device__ __forceinline__ void sweep_tile(const double* __restrict__ U, double*
         } else {
           dU[vi] = k4;
         }
+        if (MODE == kAxpy && KEEP && sa.y3Out) {
+          // the stage-3 input u + (3/4 dt) k2 of a recomputed step
+          // (rk.cpp:104), formed as the elementwise combination forms it
+          sa.y3Out[vi] = P.u.at(v, gn) + sa.a3 * k4;
+        }
         if (MODE == kErr) {
           // rk.cpp:119-129 for this entry
           const double u0 = pick(c0, v), un = sU[v][t];
