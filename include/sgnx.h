@@ -152,6 +152,14 @@ int sgnx_bathymetry(const sgnx_solver* s, double* Hx, double* Hy);
  * cannot join a batch or be autotuned). */
 int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows);
 
+/* The RHS kernel configuration in use (no counterpart in the reference; for
+ * callers that record or reproduce a run's settings): the variant and tile
+ * height of sgnx_set_rhs_kernel, and the row sweep's waves mask (bit k set:
+ * launch kind k = plain, stage 2, stage 3, k4 built for three waves per SIMD),
+ * as set at creation (SGNX_SWEEP_WAVES3, decimal or 0x-hexadecimal) or by
+ * sgnx_autotune. Any output pointer may be null. */
+int sgnx_get_rhs_kernel(const sgnx_solver* s, int* variant, int* rows, int* waves3);
+
 /* Launch timing (instrumentation, no counterpart in the reference): while
  * enabled, every RHS launch of this handle is bracketed by HIP events on the
  * handle's stream. sgnx_rhs_timing waits for the stream, fills

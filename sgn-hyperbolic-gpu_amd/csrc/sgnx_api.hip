@@ -668,6 +668,14 @@ int sgnx_set_rhs_kernel(sgnx_solver* s, int variant, int rows) {
   return SGNX_OK;
 }
 
+int sgnx_get_rhs_kernel(const sgnx_solver* s, int* variant, int* rows, int* waves3) {
+  if (!s) return SGNX_EINVAL;
+  if (variant) *variant = s->g.variant;
+  if (rows) *rows = s->g.rows;
+  if (waves3) *waves3 = s->g.waves3;
+  return SGNX_OK;
+}
+
 int sgnx_enable_timing(sgnx_solver* s, int on) {
   if (!s) return SGNX_EINVAL;
   s->timing = on != 0;

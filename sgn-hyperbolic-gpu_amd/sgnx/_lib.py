@@ -95,6 +95,7 @@ SIGNATURES = [
     ("sgnx_energy", C.c_int, [C.c_void_p, _dp]),
     ("sgnx_bathymetry", C.c_int, [C.c_void_p, _dp, _dp]),
     ("sgnx_set_rhs_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("sgnx_get_rhs_kernel", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("sgnx_enable_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("sgnx_rhs_timing", C.c_int, [C.c_void_p, C.POINTER(C.c_long), _dp]),
     ("sgnx_eta_max", C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _dp,

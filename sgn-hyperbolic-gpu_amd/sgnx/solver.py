@@ -271,6 +271,13 @@ class SGN2D:
         """0: one workgroup per 5-element row tile; 1: row sweep of `rows` rows."""
         L.check(self.lib.sgnx_set_rhs_kernel(self.h, int(variant), int(rows)), self.h)
 
+    def rhs_kernel(self):
+        """(variant, rows, waves3) in use: the kernel variant, the row sweep's
+        tile height and its waves mask (bit k: launch kind k at three waves)."""
+        v, r, w = C.c_int(), C.c_int(), C.c_int()
+        L.check(self.lib.sgnx_get_rhs_kernel(self.h, C.byref(v), C.byref(r), C.byref(w)), self.h)
+        return v.value, r.value, w.value
+
     # ---- diagnostics (eta = h - H) ---------------------------------------------
     def etaMax(self, window=None):
         """(max eta, node index) over the nodes strictly inside window =

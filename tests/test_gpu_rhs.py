@@ -217,6 +217,29 @@ def test_rhs_device_pointers_and_determinism():
         d.free()
 
 
+def test_kernel_settings_read_back(monkeypatch):
+    """sgnx_get_rhs_kernel reports the settings in use: the waves mask from
+    SGNX_SWEEP_WAVES3 at creation (decimal or 0x-hexadecimal; 0x2 without
+    it), the tile height sgnx_set_rhs_kernel sets and what the autotuner
+    chose."""
+    import sgnx
+    from sgnx import cases
+    case = cases.gate(Nx=8, Ny=6)
+    for text, want in (("0x3", 0x3), ("10", 0xA), ("0xF", 0xF), ("2", 0x2), ("0", 0x0)):
+        monkeypatch.setenv("SGNX_SWEEP_WAVES3", text)
+        with sgnx.SGN2D(**case.solver_kwargs()) as s:
+            assert s.rhs_kernel()[2] == want, (text, s.rhs_kernel())
+    monkeypatch.delenv("SGNX_SWEEP_WAVES3")
+    with sgnx.SGN2D(**case.solver_kwargs()) as s:
+        variant, rows, waves3 = s.rhs_kernel()
+        assert variant == 1 and rows >= 1 and waves3 == 0x2
+        s.set_rhs_kernel(1, 8)
+        assert s.rhs_kernel() == (1, 8, 0x2)
+        s.upload(case.U0)
+        r, w = s.autotune(1)
+        assert s.rhs_kernel() == (1, r, w)
+
+
 @pytest.mark.parametrize("ci", range(3))
 def test_wave_and_gradient_builds_give_the_same_bits(ci, monkeypatch):
     """The builds a launch kind can take (two or three waves per SIMD, Hx, Hy
@@ -237,9 +260,11 @@ def test_wave_and_gradient_builds_give_the_same_bits(ci, monkeypatch):
     finals = []
     for mask in (0x0, 0x2, 0xF, 0x1, 0xA):
         for hxr in ("1", "0"):
-            monkeypatch.setenv("SGNX_SWEEP_WAVES3", str(mask))
+            # the mask as decimal or as 0x-hexadecimal text, both read alike
+            monkeypatch.setenv("SGNX_SWEEP_WAVES3", f"{mask:#x}" if hxr == "1" else str(mask))
             monkeypatch.setenv("SGNX_SWEEP_HXR", hxr)
             s = sgnx.SGN2D(**case.solver_kwargs())
+            assert s.rhs_kernel()[2] == mask, (mask, hxr, s.rhs_kernel())
             s.upload(case.U0)
             st = s.integrateAdaptive(0.0, 1e9, dt0, sgnx.RkOpts(**opts))
             assert (st.accepted, st.rejected) == (sto.accepted, sto.rejected), (mask, hxr)
