@@ -525,8 +525,9 @@ int sgnx_batch_autotune(sgnx_batch* b, int reps, int* rows_out, int* waves3_out)
 
 // The reverse sweeps of every member's tape in lockstep, aligned at the last
 // step: iteration k reverses step N_m - 1 - k of each member that has it, with
-// one launch per kind over all of them (the gauge cotangents, the two plain
-// RHS recomputations, the two stage combinations, the three products), each
+// one launch per kind over all of them (the gauge cotangents, the stage
+// recomputation of members whose tape keeps no stage inputs: R(u), then the
+// stage-2 launch that writes y2 and forms y3; the three products), each
 // member on its own buffers. The arithmetic is that of sgnx_adjoint on each
 // member alone, so the cotangents are bit for bit the same. Every member's
 // tape must hold every step's snapshot. When every member alone gives each
