@@ -185,8 +185,7 @@ int sgnx_create(const sgnx_desc* d, sgnx_solver** out) {
   if (const char* v = std::getenv("SGNX_RHS_VARIANT")) g.variant = std::atoi(v) == 0 ? 0 : 1;
   if (const char* r = std::getenv("SGNX_SWEEP_ROWS")) g.rows = std::max(1, std::atoi(r));
   g.waves3 = 0x2;   // stage 2 at 3 waves/SIMD, the other kinds at 2 (fastest at 1024x512)
-  // a bit mask, decimal or 0x-prefixed hexadecimal ("2" and "0x2" alike)
-  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) g.waves3 = (int)std::strtol(w, nullptr, 0) & 0xf;
+  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) g.waves3 = waves_mask_from_text(w);
   for (int k = 0; k < 4; ++k) g.bc[k] = d->bc[k];
   Phys& ph = s->ph;
   ph.g = d->par.g; ph.c = d->par.c; ph.gamma = d->par.gamma; ph.nManning = d->par.nManning;

@@ -206,7 +206,7 @@ int sgnx_batch_create(sgnx_solver* const* members, int n, sgnx_batch** out) {
     sgnx_batch_destroy(b);
     return rc;
   }
-  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) b->waves3 = (int)std::strtol(w, nullptr, 0) & 0xf;
+  if (const char* w = std::getenv("SGNX_SWEEP_WAVES3")) b->waves3 = waves_mask_from_text(w);
   else b->waves3 = members[0]->g.waves3;
   *out = b;
   return SGNX_OK;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdlib>
 
 #include "sgnx_fastdiv.h"
 #include "sgnx_phys.h"
@@ -29,6 +30,13 @@ struct OpTables {
 // 128-work-item workgroup (125 node items, two waves), so a compute unit holds
 // four workgroups of the two-wave builds and six of the three-wave one.
 constexpr int kStripW = 5;
+
+// The waves mask of SGNX_SWEEP_WAVES3 (host): 0x-prefixed hexadecimal or
+// decimal text ("0x2" and "2" alike; a leading 0 alone is not octal), low 4 bits.
+inline int waves_mask_from_text(const char* w) {
+  const bool hex = w[0] == '0' && (w[1] == 'x' || w[1] == 'X');
+  return (int)std::strtol(w, nullptr, hex ? 16 : 10) & 0xf;
+}
 
 // Bijective remap of workgroup ids so that XCD x (ids x, x+8, x+16, ...)
 // receives the contiguous tile range [start_x, start_x + count_x).

@@ -219,13 +219,14 @@ def test_rhs_device_pointers_and_determinism():
 
 def test_kernel_settings_read_back(monkeypatch):
     """sgnx_get_rhs_kernel reports the settings in use: the waves mask from
-    SGNX_SWEEP_WAVES3 at creation (decimal or 0x-hexadecimal; 0x2 without
-    it), the tile height sgnx_set_rhs_kernel sets and what the autotuner
-    chose."""
+    SGNX_SWEEP_WAVES3 at creation (decimal, a leading zero included, or
+    0x-hexadecimal; 0x2 without it), the tile height sgnx_set_rhs_kernel sets
+    and what the autotuner chose."""
     import sgnx
     from sgnx import cases
     case = cases.gate(Nx=8, Ny=6)
-    for text, want in (("0x3", 0x3), ("10", 0xA), ("0xF", 0xF), ("2", 0x2), ("0", 0x0)):
+    for text, want in (("0x3", 0x3), ("10", 0xA), ("010", 0xA), ("0xF", 0xF), ("2", 0x2),
+                       ("0", 0x0)):
         monkeypatch.setenv("SGNX_SWEEP_WAVES3", text)
         with sgnx.SGN2D(**case.solver_kwargs()) as s:
             assert s.rhs_kernel()[2] == want, (text, s.rhs_kernel())
