@@ -615,6 +615,53 @@ def test_batched_adjoint_matches_solo_bitwise():
         s.close()
 
 
+def test_batched_adjoint_of_sweep_members_matches_oracle_differences():
+    """The batched sweep on members of the production sweep (seeded shoal,
+    per-point bathymetry noise, inflow and walls, uneven sizes and step
+    counts), one tape without kept stages and one with them: each member's
+    gradient of <Ubar, U_N> + <Gbar, G> against central differences of the
+    oracle's replay of its own taped steps, along a random direction in the
+    initial state and one in the still depth."""
+    import sgnx
+    from sgnx import cases
+    cs = [cases.sweep_member(11, 20), cases.sweep_member(12, 32)]
+    steps = [5, 4]
+    keeps = [False, True]
+    rng = np.random.default_rng(404)
+    ss, runs = [], []
+    for c, k, keep in zip(cs, steps, keeps):
+        theta = c.depth["Hnod"].copy()
+        U0 = c.U0 + 1e-3 * rng.uniform(-1, 1, c.U0.size) * np.repeat([1, 0.5, 0.5, 0.2, 0.2], c.nd)
+        nodes = rng.choice(c.nd, size=5, replace=False).astype(np.int64)
+        s, UN, G = _taped_run(c, theta, U0, nodes, k, 1 << 20, keep)
+        assert s.tapeStages() == (k if keep else 0)
+        ub, gb = rng.normal(size=UN.size), rng.normal(size=G.shape)
+        runs.append((c, theta, U0, nodes, s.tapeSteps(), ub, gb))
+        ss.append(s)
+    b = sgnx.Batch(ss)
+    res = b.adjoint([r[5] for r in runs], [r[6] for r in runs])
+    for (U0b, thb), (c, theta, U0, nodes, (tt, dts), ub, gb) in zip(res, runs):
+        def loss(U0x, thx):
+            o = _oracle(c, thx)
+            U = U0x.copy()
+            rows = []
+            for t, dt in zip(tt, dts):
+                U = o.bs3_step(U, t, dt)
+                rows.append(U[nodes] - thx[nodes])
+            o.close()
+            return ub @ U + np.sum(gb * np.array(rows))
+
+        dU = rng.uniform(-1.0, 1.0, U0.size) * np.repeat([0.05, 0.05, 0.05, 0.02, 0.02], c.nd)
+        fd = (loss(U0 + EPS * dU, theta) - loss(U0 - EPS * dU, theta)) / (2 * EPS)
+        assert _rel(U0b @ dU, fd) < TOL, (c.name, U0b @ dU, fd)
+        dth = rng.uniform(-1.0, 1.0, c.nd) * 0.01
+        fdt = (loss(U0, theta + EPS * dth) - loss(U0, theta - EPS * dth)) / (2 * EPS)
+        assert _rel(thb @ dth, fdt) < TOL, (c.name, thb @ dth, fdt)
+    b.close()
+    for s in ss:
+        s.close()
+
+
 @pytest.mark.parametrize("keep", [(), (0, 2)])
 def test_batched_forward_records_the_solo_tapes(keep):
     """Members recording tapes inside a batched adaptive run (graph on) get the
