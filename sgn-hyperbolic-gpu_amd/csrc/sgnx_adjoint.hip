@@ -586,17 +586,38 @@ k_vjp_elem_batch(const VjpMember* __restrict__ mem, const long long* __restrict_
   vjp_elem_tile(mem[m].va, mem[m].g, mem[m].ph, mem[m].gt, (int)(w - tileStart[m]));
 }
 
-// the gauge cotangents of one log row per member (one block per member, one
-// thread, gauge order)
+// lam[node] += Gb[k], thetaBar[node] -= Gb[k] for the n gauges of one log row,
+// by one block: the thread of a node's first gauge adds that node's gauges in
+// gauge order, so every node receives the sums a sequential scan over the
+// gauges forms (a node may carry several gauges), bit for bit, and distinct
+// nodes proceed in parallel instead of one dependent load after another.
+__device__ __forceinline__ void gauge_bar_block(const double* __restrict__ Gb,
+                                                const long long* __restrict__ nodes, int n,
+                                                double* __restrict__ lam,
+                                                double* __restrict__ thetaBar) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const long long i = nodes[k];
+    bool first = true;
+    for (int j = 0; j < k && first; ++j) first = nodes[j] != i;
+    if (!first) continue;
+    double l = lam[i];
+    double th = thetaBar ? thetaBar[i] : 0.0;
+    for (int j = k; j < n; ++j) {
+      if (nodes[j] != i) continue;
+      l += Gb[j];
+      th -= Gb[j];
+    }
+    lam[i] = l;
+    if (thetaBar) thetaBar[i] = th;
+  }
+}
+
+// the gauge cotangents of one log row per member (one block per member)
 __global__ void k_gauge_bar_batch(const GaugeMember* __restrict__ gm, int nMem) {
-  if (threadIdx.x != 0 || (int)blockIdx.x >= nMem) return;
+  if ((int)blockIdx.x >= nMem) return;
   const GaugeMember q = gm[blockIdx.x];
   if (!q.Gb) return;
-  for (int k = 0; k < q.n; ++k) {
-    const long long i = q.nodes[k];
-    q.lam[i] += q.Gb[k];
-    if (q.thetaBar) q.thetaBar[i] -= q.Gb[k];
-  }
+  gauge_bar_block(q.Gb, q.nodes, q.n, q.lam, q.thetaBar);
 }
 
 // ---- still depth from theta -------------------------------------------------------
@@ -630,16 +651,11 @@ k_face_depth(const double* __restrict__ th, int Nx, int Ny, int perX, int perY,
   }
 }
 
-// lam_h[node] += Gb[k], thetaBar[node] -= Gb[k] for the gauges of one log row,
-// in gauge order (one thread: repeated nodes add in a fixed order)
+// the gauge cotangents of one log row (one block)
 __global__ void k_gauge_bar(const double* __restrict__ Gb, const long long* __restrict__ nodes,
                             int n, double* __restrict__ lam, double* __restrict__ thetaBar) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int k = 0; k < n; ++k) {
-    const long long i = nodes[k];
-    lam[i] += Gb[k];
-    if (thetaBar) thetaBar[i] -= Gb[k];
-  }
+  if (blockIdx.x != 0) return;
+  gauge_bar_block(Gb, nodes, n, lam, thetaBar);
 }
 
 }  // namespace

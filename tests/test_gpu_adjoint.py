@@ -211,6 +211,50 @@ def test_adjoint_of_adaptive_run_matches_finite_differences(ci):
     s.close()
 
 
+def test_repeated_gauge_nodes_in_the_adjoint():
+    """Gauges may share a node: each gauge's cotangent reaches that node once,
+    in gauge order (the gauge kernel sums a node's gauges on one thread). The
+    gradient matches central differences of the oracle's replay with every
+    gauge counted, duplicates included, and one more step through the
+    batched sweep gives the same bits."""
+    import sgnx
+    c = _cases()[2]
+    rng = np.random.default_rng(515)
+    theta = c.depth["Hnod"] + 0.002 * rng.uniform(-1, 1, c.nd)
+    U0 = c.U0 + 1e-3 * rng.uniform(-1, 1, c.U0.size) * np.repeat([1, 0.5, 0.5, 0.2, 0.2], c.nd)
+    a, b_, d = rng.choice(c.nd, size=3, replace=False)
+    nodes = np.array([a, b_, a, d, a, b_, d, a], dtype=np.int64)
+    steps = 4
+    s, UN, G = _taped_run(c, theta, U0, nodes, steps, 1 << 20)
+    Ubar = rng.normal(size=UN.size)
+    Gbar = rng.normal(size=G.shape)
+    U0b, thb = s.adjoint(Ubar, Gbar)
+    tt, dts = s.tapeSteps()
+
+    def loss(U0x, thx):
+        o = _oracle(c, thx)
+        U = U0x.copy()
+        rows = []
+        for t, dt in zip(tt, dts):
+            U = o.bs3_step(U, t, dt)
+            rows.append(U[nodes] - thx[nodes])
+        o.close()
+        return Ubar @ U + np.sum(Gbar * np.array(rows))
+
+    dU = rng.uniform(-1.0, 1.0, U0.size) * np.repeat([0.05, 0.05, 0.05, 0.02, 0.02], c.nd)
+    fd = (loss(U0 + EPS * dU, theta) - loss(U0 - EPS * dU, theta)) / (2 * EPS)
+    assert _rel(U0b @ dU, fd) < TOL, (U0b @ dU, fd)
+    dth = rng.uniform(-1.0, 1.0, c.nd) * 0.05
+    fdt = (loss(U0, theta + EPS * dth) - loss(U0, theta - EPS * dth)) / (2 * EPS)
+    assert _rel(thb @ dth, fdt) < TOL, (thb @ dth, fdt)
+    bt = sgnx.Batch([s])
+    (U0b2, thb2), = bt.adjoint([Ubar], [Gbar])
+    assert_bitwise(U0b2, U0b, "U0bar, batched sweep, repeated gauge nodes")
+    assert_bitwise(thb2, thb, "thetaBar, batched sweep, repeated gauge nodes")
+    bt.close()
+    s.close()
+
+
 def _recompute_cost(L, c):
     """Forward steps an optimal schedule needs to reverse L steps from a held
     state with c free slots (dynamic programming over the first checkpoint)."""
