@@ -76,7 +76,8 @@ def _desc(sgnx_mod, P=4, Nx=4, Ny=4, lL=None, bc=(0, 0, 0, 0)):
 
 @pytest.mark.parametrize("change,code", [
     (dict(P=3), 4), (dict(Nx=1), 1), (dict(bc=(0, 1, 0, 0)), 1), (dict(bc=(1, 1, 2, 1)), 4),
-    (dict(bc=(2, 1, 1, 1)), 1), (dict(lL=np.array([1.0, 0.0, 0.0, 0.0, 1e-300])), 4)])
+    (dict(bc=(2, 1, 1, 1)), 1), (dict(lL=np.array([1.0, 0.0, 0.0, 0.0, 1e-300])), 4),
+    (dict(Nx=4096, Ny=2622), 4)])   # 268,492,800 nodes per variable: over the 2^28 the kernels index
 def test_descriptor_validation_reports_errors(sgnx_mod, change, code):
     from sgnx import _lib as L
     d, keep = _desc(sgnx_mod, **change)
